@@ -1,0 +1,172 @@
+/*
+ * ppsk.h -- C ABI of the MI355X (gfx950) core/accessory distance library
+ * (libppsk.so).  Plain C: opaque handles, pointers and sizes; no C++ types
+ * and no exceptions cross this boundary.  Every call returns PPSK_OK or an
+ * error code; ppsk_last_error() gives the message of the calling thread's
+ * last failure.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the pp-sketchlib source tree).  The reference's C++ layer
+ * (query_db_cuda / query_db_sparse_cuda, src/gpu/gpu_api.cpp:352-531) drives
+ * its CUDA internals through src/gpu/gpu.hpp:36-73; a drop-in host layer
+ * drives this ABI from the same place instead.
+ *
+ * Sketch words are passed in the reference's own layout: for one sample and
+ * one k-mer length, sketchsize64 * bbits uint64 words, word w / bit layer l at
+ * index w * bbits + l (src/sketch/sketch.cpp:82-92).  bbits must be 14
+ * (src/reference.hpp:18, the only value the reference sketcher writes).
+ */
+#ifndef PPSK_H
+#define PPSK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PPSK_OK 0
+#define PPSK_ERR_ARG 1     /* invalid argument / incompatible sketches      */
+#define PPSK_ERR_HIP 2     /* HIP runtime or kernel failure                  */
+#define PPSK_ERR_MEMORY 3  /* device memory exhausted                        */
+#define PPSK_ERR_FIT 4     /* k-mer regression had < 2 valid points          */
+#define PPSK_ERR_STATE 5   /* operation not valid in the current state       */
+
+/* Random-match correction modes (src/random/random_match.hpp:25). */
+#define PPSK_RANDOM_NONE 0       /* RandomMC(): r = 0                          */
+#define PPSK_RANDOM_EQUAL_BASE 1 /* RandomMC(use_rc): csrs closed form         */
+#define PPSK_RANDOM_MC 2         /* Monte Carlo table, per-sample clusters     */
+
+/* Dense output kinds. */
+#define PPSK_OUT_CORE_ACC 0 /* float32 [rows][2] (core, accessory)            */
+#define PPSK_OUT_JACCARD 1  /* float32 [rows][n_k] random-corrected Jaccards  */
+#define PPSK_OUT_COUNTS 2   /* uint16  [rows][n_k] raw bin-match counts        */
+
+typedef struct ppsk_set ppsk_set; /* resident sketch set on one device */
+
+/* Message of this thread's last failed call ("" if none). */
+const char *ppsk_last_error(void);
+
+/* Library version, e.g. "1.0.0". */
+const char *ppsk_version(void);
+
+/*
+ * Select and initialise a device; report free/total HBM and LDS per
+ * workgroup.  Replaces initialise_device (src/gpu/dist.cu:373-406,
+ * declared src/gpu/gpu.hpp:36).
+ */
+int ppsk_device_init(int device_id, size_t *mem_free, size_t *mem_total,
+                     size_t *shared_size);
+
+/*
+ * Create a resident sketch set of n_samples samples with the given k-mer
+ * lengths (ascending) and sketch size.  Replaces the flattening and
+ * host->device copies of flatten_by_bins / flatten_by_samples and
+ * DeviceMemory (src/gpu/gpu.hpp:38-49, src/gpu/device_memory.cu:6-65): the
+ * set is kept in HBM in the library's own stripe layout and reused across
+ * queries.
+ */
+int ppsk_set_create(int device_id, int64_t n_samples, int32_t n_k,
+                    const int32_t *klist, int32_t sketchsize64, int32_t bbits,
+                    ppsk_set **out);
+
+/* Upload samples [first, first + count): raw is [count][n_k][s64 * bbits]. */
+int ppsk_set_upload(ppsk_set *set, int64_t first, int64_t count,
+                    const uint64_t *raw);
+
+/*
+ * Per-sample random-match inputs.  lengths (may be NULL when the equal-base
+ * mode is unused): sequence lengths, from which the library precomputes the
+ * csrs terms j(len) = 1 - (1 - 2^-((k-1)*2+rc))^len per k
+ * (src/random/random_match.cpp:318-336; k >= 33 is rejected, the reference
+ * shift is undefined there).  clusters (may be NULL): Monte Carlo cluster id
+ * per sample (RandomMC::lookup_array, random_match.cpp:254-273).
+ */
+int ppsk_set_meta(ppsk_set *set, const uint64_t *lengths, int32_t use_rc,
+                  const uint16_t *clusters);
+
+int ppsk_set_free(ppsk_set *set);
+
+int64_t ppsk_set_size(const ppsk_set *set);
+
+/*
+ * Dense distances.  query == NULL (or == ref): self mode, rows are the
+ * condensed upper triangle, row = square_to_condensed(i, j, n) for i < j.
+ * Otherwise rows are query-major, row = q * n_ref + r.  The Jaccard
+ * denominator comes from the row sample (r1 = i in self mode, r1 = query in
+ * query mode, src/api.cpp:195-268).  Random correction: mode and, for
+ * PPSK_RANDOM_MC, a [n_k][C][C] float table indexed
+ * [cluster(r1 of random_match)][cluster(other)] with zeros where
+ * k >= max_k (random_match.cpp:185-206).  tolerance: the regression validity
+ * threshold (src/dist/linear_regression.cpp:22).
+ * out: host memory (out_on_device = 0) or device memory from
+ * ppsk_device_alloc (out_on_device = 1).
+ * Pairs whose regression window has < 2 points get (0, 0), are counted in
+ * *n_fail (may be NULL) with the smallest such row in *first_fail_row, and
+ * the call returns PPSK_ERR_FIT after writing every other row.
+ * Replaces dispatchDists (src/gpu/dist.cu:408-489, src/gpu/gpu.hpp:51-61).
+ */
+int ppsk_dense(ppsk_set *ref, ppsk_set *query, int32_t random_mode,
+               int32_t n_clusters, const float *mc_table, double tolerance,
+               int32_t out_kind, void *out, int32_t out_on_device,
+               uint64_t *n_fail, int64_t *first_fail_row);
+
+/*
+ * Self kNN, as query_db_sparse: for every sample i the kNN smallest of
+ * column dist_col (0 core, 1 accessory) over all j (i == j counts as +inf),
+ * ascending per row, i_vec/j_vec/dists of n * kNN entries.  Selection and
+ * tie order reproduce the reference's std::priority_queue scan over j
+ * (src/api.cpp:319-432).  Requires 1 <= kNN <= n.
+ * Replaces sparseDists (src/gpu/dist.cu:495-791, src/gpu/gpu.hpp:63-73).
+ */
+int ppsk_knn(ppsk_set *ref, int32_t random_mode, int32_t n_clusters,
+             const float *mc_table, double tolerance, int32_t kNN,
+             int32_t dist_col, int64_t *i_vec, int64_t *j_vec, float *dists,
+             uint64_t *n_fail);
+
+/*
+ * Self threshold sparsification, as querySelfSparse with dist_cutoff > 0:
+ * upper triangle (i < j) entries of column dist_col with d < cutoff, in
+ * row-major order (src/sketchlib_bindings.cpp:159-169,
+ * src/dist/matrix_ops.cpp:38-69).  The library allocates *i_vec, *j_vec and
+ * *dists (n_out entries each); release them with ppsk_free_host.
+ */
+int ppsk_threshold(ppsk_set *ref, int32_t random_mode, int32_t n_clusters,
+                   const float *mc_table, double tolerance, int32_t dist_col,
+                   float cutoff, int64_t **i_vec, int64_t **j_vec,
+                   float **dists, int64_t *n_out, uint64_t *n_fail);
+
+void ppsk_free_host(void *ptr);
+
+/* Device memory for out_on_device outputs. */
+int ppsk_device_alloc(int device_id, size_t bytes, void **ptr);
+int ppsk_device_free(void *ptr);
+int ppsk_copy_to_host(void *dst, const void *device_src, size_t bytes);
+
+/*
+ * Timing of the last dense/kNN/threshold call on this thread, measured with
+ * HIP events on the launch stream: summed duration of the distance-tile
+ * kernel launches and their number.
+ */
+int ppsk_last_timing(double *kernel_ms, int64_t *n_launches, int64_t *n_items);
+
+/*
+ * Seeded synthetic sketches (no reference counterpart: a generator for tests
+ * and benchmarks, writing the reference word layout directly).  Bins evolve
+ * down a random binary tree: a child keeps its parent's 14-bit bin value with
+ * probability (1 - dc)^k (1 - da), else draws a fresh uniform value; edge
+ * rates dc ~ U[0, dcore_max / D], da ~ U[0, dacc_max / D], D = ceil(log2 n).
+ * raw: [n][n_k][s64 * 14] uint64; lengths: [n] (may be NULL); base_freq:
+ * [n][4] ACGT (may be NULL).  Deterministic for any n_threads (0 = all).
+ */
+int ppsk_synth_sketches(int64_t n, int32_t n_k, const int32_t *klist,
+                        int32_t sketchsize64, uint64_t seed, double dcore_max,
+                        double dacc_max, double length_mean, uint64_t *raw,
+                        uint64_t *lengths, double *base_freq, int32_t n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PPSK_H */

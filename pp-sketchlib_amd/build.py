@@ -1,0 +1,103 @@
+"""Build the native parts in-tree.
+
+  libppsk.so            HIP kernels + C ABI (include/ppsk.h), gfx950
+  pp_sketchlib*.so      pybind11 module with the reference's Python API,
+                        linked against libppsk.so and HDF5
+
+Run `python build.py` (or call build_all()).  Every call recompiles.
+"""
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+HDF5_PREFIX = os.environ.get("PPSK_HDF5_PREFIX", "/opt/conda")
+
+LIB = os.path.join(HERE, "libppsk.so")
+
+HIP_SOURCES = ["dist_kernels.hip", "select_kernels.hip", "ppsk_capi.cpp"]
+HOST_SOURCES = ["synth.cpp"]
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    # IEEE double for the Jaccard / random-match arithmetic: no FMA contraction
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-Wno-unused-value",
+]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines()
+                    if "packed-fp32-ops' is not a recognized feature" not in l)
+    if out.strip():
+        print(out)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+
+
+def build_lib():
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(HERE, "_build", os.path.splitext(src)[0] + ".o")
+        os.makedirs(os.path.dirname(obj), exist_ok=True)
+        _run([HIPCC, "-x", "hip"] + HIP_FLAGS + ["-I", INCLUDE, "-c",
+                                                  os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(HERE, "_build", os.path.splitext(src)[0] + ".o")
+        _run(["g++", "-O3", "-std=c++17", "-fPIC", "-pthread", "-I", INCLUDE, "-c",
+              os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def module_path():
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    return os.path.join(HERE, "pp_sketchlib" + suffix)
+
+
+def build_module():
+    import pybind11
+
+    out = module_path()
+    tmp = out + ".tmp"
+    py_inc = sysconfig.get_paths()["include"]
+    srcs = [os.path.join(CSRC, "host", f) for f in
+            ("sketch_db.cpp", "random_match.cpp", "api.cpp", "matrix_ops.cpp", "bindings.cpp")]
+    cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
+           "-ffp-contract=off", "-fvisibility=hidden",
+           "-I", INCLUDE, "-I", CSRC, "-I", pybind11.get_include(), "-I", py_inc,
+           "-I", os.path.join(HDF5_PREFIX, "include"),
+           "-DPPSK_VERSION_INFO=\"2.1.5\""] + srcs + [
+           "-L", HERE, "-lppsk",
+           "-L", os.path.join(HDF5_PREFIX, "lib"), "-l:libhdf5.so.103",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(HDF5_PREFIX, "lib"),
+           "-o", tmp]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_all(module=True):
+    build_lib()
+    if module and os.path.isdir(os.path.join(CSRC, "host")):
+        build_module()
+
+
+if __name__ == "__main__":
+    build_all(module="--lib-only" not in sys.argv)

@@ -1,0 +1,688 @@
+// C-ABI implementation (include/ppsk.h): resident sketch sets, device
+// workspaces, tile scheduling and the dense / kNN / threshold entry points.
+// Compiled by hipcc together with dist_kernels.hip and select_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ppsk.h"
+#include "ppsk_internal.h"
+#include "select_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+thread_local double g_last_ms = 0.0;
+thread_local int64_t g_last_launches = 0;
+thread_local int64_t g_last_items = 0;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      return fail(_e == hipErrorOutOfMemory ? PPSK_ERR_MEMORY : PPSK_ERR_HIP,      \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));              \
+    }                                                                              \
+  } while (0)
+
+// Per-device state: one stream, cached workspaces grown on demand.
+struct DeviceCtx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void* counts = nullptr;
+  size_t counts_bytes = 0;
+  uint32_t* tile_done = nullptr;
+  size_t tile_done_n = 0;
+  ppsk::Tile* tiles = nullptr;
+  size_t tiles_n = 0;
+  unsigned long long* fail = nullptr;  // [0] count, [1] first row
+  void* out = nullptr;
+  size_t out_bytes = 0;
+  std::mutex mu;
+};
+
+std::mutex g_ctx_mu;
+std::vector<DeviceCtx*> g_ctx;
+
+int get_ctx(int device, DeviceCtx** out) {
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) {
+    return fail(PPSK_ERR_ARG, "device_id " + std::to_string(device) + " out of range");
+  }
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  if ((int)g_ctx.size() < n) g_ctx.resize(n, nullptr);
+  if (!g_ctx[device]) {
+    HIP_TRY(hipSetDevice(device));
+    auto* c = new DeviceCtx();
+    c->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&c->ev0));
+    HIP_TRY(hipEventCreate(&c->ev1));
+    HIP_TRY(hipMalloc(&c->fail, 2 * sizeof(unsigned long long)));
+    g_ctx[device] = c;
+  }
+  *out = g_ctx[device];
+  HIP_TRY(hipSetDevice(device));
+  return PPSK_OK;
+}
+
+int grow(void** ptr, size_t* have, size_t need) {
+  if (*have >= need && *ptr) return PPSK_OK;
+  if (*ptr) HIP_TRY(hipFree(*ptr));
+  *ptr = nullptr;
+  *have = 0;
+  HIP_TRY(hipMalloc(ptr, need));
+  *have = need;
+  return PPSK_OK;
+}
+
+int grow_tile_done(DeviceCtx* c, size_t n) {
+  if (c->tile_done_n >= n && c->tile_done) return PPSK_OK;
+  if (c->tile_done) HIP_TRY(hipFree(c->tile_done));
+  c->tile_done = nullptr;
+  c->tile_done_n = 0;
+  HIP_TRY(hipMalloc(&c->tile_done, n * sizeof(uint32_t)));
+  // counters are left at zero by every tile's last workgroup afterwards
+  HIP_TRY(hipMemset(c->tile_done, 0, n * sizeof(uint32_t)));
+  c->tile_done_n = n;
+  return PPSK_OK;
+}
+
+}  // namespace
+
+struct ppsk_set {
+  int device;
+  int64_t n;
+  int32_t n_k;
+  std::vector<int32_t> klist;
+  int32_t s64;
+  int32_t bbits;
+  int32_t nb;
+  uint32_t* stripes = nullptr;  // [n_k][nb][s64][28][128]
+  double* jv = nullptr;         // [n][n_k]
+  uint16_t* clusters = nullptr; // [n]
+  bool has_lengths = false;
+  int32_t use_rc = 1;
+  int32_t max_cluster = -1;     // largest cluster id uploaded
+};
+
+namespace {
+
+int check_set(const ppsk_set* s, const char* what) {
+  if (!s) return fail(PPSK_ERR_ARG, std::string(what) + " is NULL");
+  return PPSK_OK;
+}
+
+int check_compatible(const ppsk_set* a, const ppsk_set* b) {
+  if (a->device != b->device) return fail(PPSK_ERR_ARG, "sketch sets on different devices");
+  if (a->bbits != b->bbits) return fail(PPSK_ERR_ARG, "Mismatching bbits in sketches");
+  if (a->s64 != b->s64) return fail(PPSK_ERR_ARG, "Mismatching sketchsize64 in sketches");
+  if (a->klist != b->klist) return fail(PPSK_ERR_ARG, "Mismatching k-mer lengths in sketches");
+  return PPSK_OK;
+}
+
+// Tile lists.  Self: upper-triangle tiles (at least one pair with i < j).
+// Full square / query: every tile.  Row-major order (consecutive tiles share
+// the 128-row block).
+std::vector<ppsk::Tile> make_tiles(int64_t n_rows, int64_t n_cols, int tc, bool upper,
+                                   int64_t row_begin, int64_t row_end) {
+  std::vector<ppsk::Tile> t;
+  const int64_t nbr = (n_rows + ppsk::BLK - 1) / ppsk::BLK;
+  const int64_t nct = (n_cols + tc - 1) / tc;
+  const int64_t b0 = row_begin / ppsk::BLK;
+  const int64_t b1 = std::min(nbr, (row_end + ppsk::BLK - 1) / ppsk::BLK);
+  for (int64_t bi = b0; bi < b1; ++bi) {
+    for (int64_t ct = 0; ct < nct; ++ct) {
+      if (upper && (ct + 1) * tc - 1 <= bi * ppsk::BLK) continue;
+      t.push_back({(int32_t)bi, (int32_t)ct});
+    }
+  }
+  return t;
+}
+
+struct RandomIn {
+  int32_t mode;
+  int32_t n_clusters;
+  const float* table;
+};
+
+int check_random(const ppsk_set* rows, const ppsk_set* cols, const RandomIn& rnd) {
+  if (rnd.mode == PPSK_RANDOM_EQUAL_BASE) {
+    if (!rows->has_lengths || !cols->has_lengths) {
+      return fail(PPSK_ERR_ARG, "equal-base random correction needs sample lengths (ppsk_set_meta)");
+    }
+  } else if (rnd.mode == PPSK_RANDOM_MC) {
+    if (!rows->clusters || !cols->clusters) {
+      return fail(PPSK_ERR_ARG, "Monte Carlo random correction needs cluster ids (ppsk_set_meta)");
+    }
+    if (rnd.n_clusters < 1 || !rnd.table) {
+      return fail(PPSK_ERR_ARG, "Monte Carlo random correction needs a table");
+    }
+    if (rows->max_cluster >= rnd.n_clusters || cols->max_cluster >= rnd.n_clusters) {
+      return fail(PPSK_ERR_ARG, "cluster id outside the Monte Carlo table");
+    }
+  } else if (rnd.mode != PPSK_RANDOM_NONE) {
+    return fail(PPSK_ERR_ARG, "unknown random_mode");
+  }
+  return PPSK_OK;
+}
+
+// Runs the distance tiles for `tiles` (chunked by count-scratch size) with
+// params p (tiles/counts/tile_done/item_base filled in here); accumulates
+// kernel time.
+int run_tiles(DeviceCtx* c, ppsk::DistParams p, const std::vector<ppsk::Tile>& tiles,
+              int rr, double* ms_acc, int64_t* launches, int64_t* items) {
+  if (tiles.empty()) return PPSK_OK;
+  const int tc = 64 * rr;
+  const size_t per_tile = (size_t)p.n_k * ppsk::TROWS * tc * sizeof(uint16_t);
+  const size_t budget = (size_t)1 << 30;  // count scratch per launch
+  size_t chunk = std::max<size_t>(1, budget / per_tile);
+  chunk = std::min(chunk, tiles.size());
+  if (int rc = grow(&c->counts, &c->counts_bytes, chunk * per_tile)) return rc;
+  if (int rc = grow_tile_done(c, chunk)) return rc;
+  void* tptr = c->tiles;
+  size_t tbytes = c->tiles_n * sizeof(ppsk::Tile);
+  if (int rc = grow(&tptr, &tbytes, tiles.size() * sizeof(ppsk::Tile))) return rc;
+  c->tiles = (ppsk::Tile*)tptr;
+  c->tiles_n = tbytes / sizeof(ppsk::Tile);
+  HIP_TRY(hipMemcpyAsync(c->tiles, tiles.data(), tiles.size() * sizeof(ppsk::Tile),
+                         hipMemcpyHostToDevice, c->stream));
+  p.counts = (uint16_t*)c->counts;
+  p.tile_done = c->tile_done;
+  for (size_t t0 = 0; t0 < tiles.size(); t0 += chunk) {
+    const size_t nt = std::min(chunk, tiles.size() - t0);
+    p.tiles = c->tiles + t0;
+    p.n_tiles = (int64_t)nt;
+    p.item_base = 0;
+    const int64_t n_items = (int64_t)nt * p.n_k;
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(ppsk::launch_dists(p, rr, n_items, c->stream));
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *ms_acc += ms;
+    *launches += 1;
+    *items += n_items;
+  }
+  return PPSK_OK;
+}
+
+void fill_common(ppsk::DistParams& p, const ppsk_set* rows, const ppsk_set* cols,
+                 const RandomIn& rnd, double tol) {
+  std::memset(&p, 0, sizeof(p));
+  p.row_stripes = rows->stripes;
+  p.col_stripes = cols->stripes;
+  p.n_rows = rows->n;
+  p.n_cols = cols->n;
+  p.nb_rows = rows->nb;
+  p.nb_cols = cols->nb;
+  p.n_k = rows->n_k;
+  p.s64 = rows->s64;
+  for (int k = 0; k < rows->n_k; ++k) p.kvals[k] = (float)rows->klist[k];
+  p.tolerance = tol;
+  p.random_mode = rnd.mode;
+  p.n_clusters = rnd.n_clusters;
+  p.jv_rows = rows->jv;
+  p.jv_cols = cols->jv;
+  p.cl_rows = rows->clusters;
+  p.cl_cols = cols->clusters;
+}
+
+int upload_table(DeviceCtx* c, const ppsk_set* s, const RandomIn& rnd, float** dtab) {
+  *dtab = nullptr;
+  if (rnd.mode != PPSK_RANDOM_MC) return PPSK_OK;
+  const size_t bytes = (size_t)s->n_k * rnd.n_clusters * rnd.n_clusters * sizeof(float);
+  HIP_TRY(hipMalloc((void**)dtab, bytes));
+  HIP_TRY(hipMemcpyAsync(*dtab, rnd.table, bytes, hipMemcpyHostToDevice, c->stream));
+  return PPSK_OK;
+}
+
+int reset_fail(DeviceCtx* c) {
+  unsigned long long init[2] = {0ull, ~0ull};
+  HIP_TRY(hipMemcpyAsync(c->fail, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  return PPSK_OK;
+}
+
+int read_fail(DeviceCtx* c, unsigned long long* cnt, unsigned long long* first) {
+  unsigned long long v[2];
+  HIP_TRY(hipMemcpyAsync(v, c->fail, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *cnt = v[0];
+  *first = v[1];
+  return PPSK_OK;
+}
+
+int default_rr() {
+  static int rr = -1;
+  if (rr < 0) {
+    const char* e = std::getenv("PPSK_TILE_RR");
+    rr = (e && std::atoi(e) == 1) ? 1 : 2;
+  }
+  return rr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ppsk_last_error(void) { return g_last_error.c_str(); }
+
+const char* ppsk_version(void) { return "1.0.0"; }
+
+int ppsk_device_init(int device_id, size_t* mem_free, size_t* mem_total, size_t* shared_size) {
+  DeviceCtx* c = nullptr;
+  if (int rc = get_ctx(device_id, &c)) return rc;
+  size_t fr = 0, tot = 0;
+  HIP_TRY(hipMemGetInfo(&fr, &tot));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device_id));
+  if (mem_free) *mem_free = fr;
+  if (mem_total) *mem_total = tot;
+  if (shared_size) *shared_size = prop.sharedMemPerBlock;
+  return PPSK_OK;
+}
+
+int ppsk_set_create(int device_id, int64_t n_samples, int32_t n_k, const int32_t* klist,
+                    int32_t sketchsize64, int32_t bbits, ppsk_set** out) {
+  if (!out) return fail(PPSK_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (n_samples < 1) return fail(PPSK_ERR_ARG, "Query with empty ref or query list!");
+  if (n_k < 1 || n_k > ppsk::MAXK || !klist) {
+    return fail(PPSK_ERR_ARG, "n_k must be in [1, " + std::to_string(ppsk::MAXK) + "]");
+  }
+  for (int i = 1; i < n_k; ++i) {
+    if (klist[i] <= klist[i - 1]) return fail(PPSK_ERR_ARG, "klist must be strictly ascending");
+  }
+  if (bbits != ppsk::BBITS) {
+    return fail(PPSK_ERR_ARG, "only bbits = 14 sketches are supported (got " +
+                                  std::to_string(bbits) + ")");
+  }
+  if (sketchsize64 < 1 || sketchsize64 > 1023) {
+    return fail(PPSK_ERR_ARG, "sketchsize64 must be in [1, 1023]");
+  }
+  DeviceCtx* c = nullptr;
+  if (int rc = get_ctx(device_id, &c)) return rc;
+  auto* s = new ppsk_set();
+  s->device = device_id;
+  s->n = n_samples;
+  s->n_k = n_k;
+  s->klist.assign(klist, klist + n_k);
+  s->s64 = sketchsize64;
+  s->bbits = bbits;
+  s->nb = (int32_t)((n_samples + ppsk::BLK - 1) / ppsk::BLK);
+  const size_t bytes = (size_t)n_k * s->nb * sketchsize64 * ppsk::HL * ppsk::BLK * sizeof(uint32_t);
+  hipError_t e = hipMalloc(&s->stripes, bytes);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(PPSK_ERR_MEMORY, std::string("sketch set allocation: ") + hipGetErrorString(e));
+  }
+  e = hipMemsetAsync(s->stripes, 0, bytes, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    hipFree(s->stripes);
+    delete s;
+    return fail(PPSK_ERR_HIP, std::string("sketch set clear: ") + hipGetErrorString(e));
+  }
+  *out = s;
+  return PPSK_OK;
+}
+
+int ppsk_set_upload(ppsk_set* s, int64_t first, int64_t count, const uint64_t* raw) {
+  if (int rc = check_set(s, "set")) return rc;
+  if (first < 0 || count < 0 || first + count > s->n) {
+    return fail(PPSK_ERR_ARG, "upload range outside the set");
+  }
+  if (count == 0) return PPSK_OK;
+  if (!raw) return fail(PPSK_ERR_ARG, "raw is NULL");
+  DeviceCtx* c = nullptr;
+  if (int rc = get_ctx(s->device, &c)) return rc;
+  const size_t per_sample = (size_t)s->n_k * s->s64 * s->bbits * sizeof(uint64_t);
+  const int64_t chunk = std::max<int64_t>(1, (int64_t)((256u << 20) / per_sample));
+  void* staging = nullptr;
+  HIP_TRY(hipMalloc(&staging, (size_t)std::min(chunk, count) * per_sample));
+  int rc = PPSK_OK;
+  for (int64_t c0 = 0; c0 < count && rc == PPSK_OK; c0 += chunk) {
+    const int64_t m = std::min(chunk, count - c0);
+    hipError_t e = hipMemcpyAsync(staging, (const char*)raw + (size_t)c0 * per_sample,
+                                  (size_t)m * per_sample, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+      e = ppsk::launch_stripe((const uint64_t*)staging, m, first + c0, s->n_k, s->s64, s->nb,
+                              s->stripes, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(PPSK_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+  }
+  hipFree(staging);
+  return rc;
+}
+
+int ppsk_set_meta(ppsk_set* s, const uint64_t* lengths, int32_t use_rc, const uint16_t* clusters) {
+  if (int rc = check_set(s, "set")) return rc;
+  DeviceCtx* c = nullptr;
+  if (int rc = get_ctx(s->device, &c)) return rc;
+  if (lengths) {
+    // csrs terms per sample and k (src/random/random_match.cpp:318-336)
+    std::vector<double> jv((size_t)s->n * s->n_k);
+    for (int kk = 0; kk < s->n_k; ++kk) {
+      const int64_t k = s->klist[kk];
+      const int64_t shift = (k - 1) * 2 + (use_rc ? 1 : 0);
+      if (k < 1 || shift >= 64) {
+        return fail(PPSK_ERR_ARG, "equal-base random correction is undefined for k = " +
+                                      std::to_string(k) + " (k must be 1..32)");
+      }
+      const size_t match_chance = (size_t)1 << shift;
+      const double base = 1 - (double)1 / match_chance;
+      for (int64_t i = 0; i < s->n; ++i) {
+        jv[(size_t)i * s->n_k + kk] = 1 - std::pow(base, (double)lengths[i]);
+      }
+    }
+    if (!s->jv) HIP_TRY(hipMalloc(&s->jv, jv.size() * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(s->jv, jv.data(), jv.size() * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    s->has_lengths = true;
+    s->use_rc = use_rc;
+  }
+  if (clusters) {
+    s->max_cluster = 0;
+    for (int64_t i = 0; i < s->n; ++i) s->max_cluster = std::max<int32_t>(s->max_cluster, clusters[i]);
+    if (!s->clusters) HIP_TRY(hipMalloc(&s->clusters, (size_t)s->n * sizeof(uint16_t)));
+    HIP_TRY(hipMemcpyAsync(s->clusters, clusters, (size_t)s->n * sizeof(uint16_t),
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return PPSK_OK;
+}
+
+int ppsk_set_free(ppsk_set* s) {
+  if (!s) return PPSK_OK;
+  hipSetDevice(s->device);
+  if (s->stripes) hipFree(s->stripes);
+  if (s->jv) hipFree(s->jv);
+  if (s->clusters) hipFree(s->clusters);
+  delete s;
+  return PPSK_OK;
+}
+
+int64_t ppsk_set_size(const ppsk_set* s) { return s ? s->n : 0; }
+
+int ppsk_dense(ppsk_set* ref, ppsk_set* query, int32_t random_mode, int32_t n_clusters,
+               const float* mc_table, double tolerance, int32_t out_kind, void* out,
+               int32_t out_on_device, uint64_t* n_fail, int64_t* first_fail_row) {
+  g_last_ms = 0;
+  g_last_launches = 0;
+  g_last_items = 0;
+  if (int rc = check_set(ref, "ref")) return rc;
+  const bool self = (query == nullptr || query == ref);
+  if (!self) {
+    if (int rc = check_compatible(ref, query)) return rc;
+  }
+  if (out_kind < PPSK_OUT_CORE_ACC || out_kind > PPSK_OUT_COUNTS) {
+    return fail(PPSK_ERR_ARG, "unknown out_kind");
+  }
+  if (out_kind == PPSK_OUT_CORE_ACC && ref->n_k < 2) {
+    return fail(PPSK_ERR_ARG, "core/accessory distances need at least two k-mer lengths");
+  }
+  if (!out) return fail(PPSK_ERR_ARG, "out is NULL");
+  const ppsk_set* rows = self ? ref : query;
+  const ppsk_set* cols = ref;
+  RandomIn rnd{random_mode, n_clusters, mc_table};
+  if (int rc = check_random(rows, cols, rnd)) return rc;
+  DeviceCtx* c = nullptr;
+  if (int rc = get_ctx(ref->device, &c)) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+
+  const int64_t n_rows_out = self ? ref->n * (ref->n - 1) / 2 : query->n * ref->n;
+  const size_t elem = out_kind == PPSK_OUT_CORE_ACC ? 2 * sizeof(float)
+                      : out_kind == PPSK_OUT_JACCARD ? ref->n_k * sizeof(float)
+                                                     : ref->n_k * sizeof(uint16_t);
+  const size_t out_bytes = (size_t)n_rows_out * elem;
+  if (n_rows_out == 0) {
+    if (n_fail) *n_fail = 0;
+    if (first_fail_row) *first_fail_row = -1;
+    return PPSK_OK;
+  }
+  void* dout = out;
+  if (!out_on_device) {
+    if (int rc = grow(&c->out, &c->out_bytes, out_bytes)) return rc;
+    dout = c->out;
+  }
+  float* dtab = nullptr;
+  if (int rc = upload_table(c, ref, rnd, &dtab)) return rc;
+
+  ppsk::DistParams p;
+  fill_common(p, rows, cols, rnd, tolerance);
+  p.self_mode = self ? 1 : 0;
+  p.full_square = 0;
+  p.out_kind = out_kind;
+  p.out = dout;
+  p.mc_table = dtab;
+  p.mc_ref_is_col = self ? 0 : 1;  // query mode: random_match(r1 = ref, q)
+  p.fail_count = c->fail;
+  p.fail_first = c->fail + 1;
+  if (int rc = reset_fail(c)) return rc;
+
+  const int rr = default_rr();
+  auto tiles = make_tiles(rows->n, cols->n, 64 * rr, self, 0, rows->n);
+  int rc = run_tiles(c, p, tiles, rr, &g_last_ms, &g_last_launches, &g_last_items);
+  if (dtab) hipFree(dtab);
+  if (rc) return rc;
+  unsigned long long nf = 0, ff = 0;
+  if (int e = read_fail(c, &nf, &ff)) return e;
+  if (!out_on_device) {
+    HIP_TRY(hipMemcpy(out, dout, out_bytes, hipMemcpyDeviceToHost));
+  }
+  if (n_fail) *n_fail = nf;
+  if (first_fail_row) *first_fail_row = nf ? (int64_t)ff : -1;
+  if (nf) {
+    return fail(PPSK_ERR_FIT, "Fitting k-mer gradient failed for " + std::to_string(nf) +
+                                  " pair(s); first at row " + std::to_string(ff));
+  }
+  return PPSK_OK;
+}
+
+int ppsk_knn(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float* mc_table,
+             double tolerance, int32_t kNN, int32_t dist_col, int64_t* i_vec, int64_t* j_vec,
+             float* dists, uint64_t* n_fail) {
+  g_last_ms = 0;
+  g_last_launches = 0;
+  g_last_items = 0;
+  if (int rc = check_set(ref, "ref")) return rc;
+  if (kNN < 1 || kNN > ref->n) {
+    return fail(PPSK_ERR_ARG, "kNN must be in [1, number of samples]");
+  }
+  if (dist_col != 0 && dist_col != 1) return fail(PPSK_ERR_ARG, "dist_col out of range");
+  if (ref->n_k < 2) {
+    return fail(PPSK_ERR_ARG, "core/accessory distances need at least two k-mer lengths");
+  }
+  if (!i_vec || !j_vec || !dists) return fail(PPSK_ERR_ARG, "output is NULL");
+  RandomIn rnd{random_mode, n_clusters, mc_table};
+  if (int rc = check_random(ref, ref, rnd)) return rc;
+  DeviceCtx* c = nullptr;
+  if (int rc = get_ctx(ref->device, &c)) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  const int64_t n = ref->n;
+  const int rr = default_rr();
+  // bands of rows: square block [band][n] float2 (core, acc)
+  int64_t band = std::max<int64_t>(ppsk::BLK, ((int64_t)(2ll << 30) / (n * 8)) / ppsk::BLK * ppsk::BLK);
+  band = std::min<int64_t>(band, (n + ppsk::BLK - 1) / ppsk::BLK * ppsk::BLK);
+  if (int rc = grow(&c->out, &c->out_bytes, (size_t)band * n * 2 * sizeof(float))) return rc;
+  float* dtab = nullptr;
+  if (int rc = upload_table(c, ref, rnd, &dtab)) return rc;
+  int64_t* d_i = nullptr;
+  int64_t* d_j = nullptr;
+  float* d_d = nullptr;
+  int rc = PPSK_OK;
+  auto cleanup = [&]() {
+    if (dtab) hipFree(dtab);
+    if (d_i) hipFree(d_i);
+    if (d_j) hipFree(d_j);
+    if (d_d) hipFree(d_d);
+  };
+  if (hipMalloc(&d_i, (size_t)band * kNN * sizeof(int64_t)) != hipSuccess ||
+      hipMalloc(&d_j, (size_t)band * kNN * sizeof(int64_t)) != hipSuccess ||
+      hipMalloc(&d_d, (size_t)band * kNN * sizeof(float)) != hipSuccess) {
+    cleanup();
+    return fail(PPSK_ERR_MEMORY, "kNN output buffers");
+  }
+  if ((rc = reset_fail(c))) {
+    cleanup();
+    return rc;
+  }
+  for (int64_t r0 = 0; r0 < n && rc == PPSK_OK; r0 += band) {
+    const int64_t r1 = std::min(n, r0 + band);
+    ppsk::DistParams p;
+    fill_common(p, ref, ref, rnd, tolerance);
+    p.self_mode = 1;
+    p.full_square = 1;
+    p.out_kind = PPSK_OUT_CORE_ACC;
+    p.out = c->out;
+    p.out_stride_rows = n;
+    p.square_row0 = r0;
+    p.mc_table = dtab;
+    p.mc_ref_is_col = 0;
+    p.fail_count = c->fail;
+    p.fail_first = c->fail + 1;
+    auto tiles = make_tiles(n, n, 64 * rr, false, r0, r1);
+    rc = run_tiles(c, p, tiles, rr, &g_last_ms, &g_last_launches, &g_last_items);
+    if (rc) break;
+    hipError_t e = ppsk::launch_knn_select((const float*)c->out, r0, r1 - r0, n, dist_col, kNN,
+                                           d_i, d_j, d_d, c->stream);
+    if (e == hipSuccess) {
+      const size_t m = (size_t)(r1 - r0) * kNN;
+      e = hipMemcpyAsync(i_vec + r0 * kNN, d_i, m * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(j_vec + r0 * kNN, d_j, m * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(dists + r0 * kNN, d_d, m * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    if (e != hipSuccess) rc = fail(PPSK_ERR_HIP, std::string("kNN select: ") + hipGetErrorString(e));
+  }
+  cleanup();
+  if (rc) return rc;
+  unsigned long long nf = 0, ff = 0;
+  if (int e = read_fail(c, &nf, &ff)) return e;
+  if (n_fail) *n_fail = nf;
+  if (nf) {
+    return fail(PPSK_ERR_FIT, "Fitting k-mer gradient failed for " + std::to_string(nf) +
+                                  " pair(s)");
+  }
+  return PPSK_OK;
+}
+
+int ppsk_threshold(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float* mc_table,
+                   double tolerance, int32_t dist_col, float cutoff, int64_t** i_vec,
+                   int64_t** j_vec, float** dists, int64_t* n_out, uint64_t* n_fail) {
+  if (int rc = check_set(ref, "ref")) return rc;
+  if (!(cutoff >= 0)) return fail(PPSK_ERR_ARG, "Distance threshold must be at least 0");
+  if (dist_col != 0 && dist_col != 1) return fail(PPSK_ERR_ARG, "dist_col out of range");
+  if (!i_vec || !j_vec || !dists || !n_out) return fail(PPSK_ERR_ARG, "output is NULL");
+  *i_vec = nullptr;
+  *j_vec = nullptr;
+  *dists = nullptr;
+  *n_out = 0;
+  const int64_t n = ref->n;
+  const int64_t rows = n * (n - 1) / 2;
+  DeviceCtx* c = nullptr;
+  if (int rc = get_ctx(ref->device, &c)) return rc;
+  float2* dd = nullptr;
+  HIP_TRY(hipMalloc(&dd, std::max<int64_t>(rows, 1) * sizeof(float2)));
+  uint64_t nf = 0;
+  int64_t ff = -1;
+  int rc = ppsk_dense(ref, nullptr, random_mode, n_clusters, mc_table, tolerance,
+                      PPSK_OUT_CORE_ACC, dd, 1, &nf, &ff);
+  if (n_fail) *n_fail = nf;
+  if (rc) {
+    hipFree(dd);
+    return rc;
+  }
+  double ms = g_last_ms;
+  int64_t nl = g_last_launches, ni = g_last_items;
+  std::lock_guard<std::mutex> lk(c->mu);
+  int64_t total = 0;
+  int64_t *d_i = nullptr, *d_j = nullptr;
+  float* d_d = nullptr;
+  hipError_t e = ppsk::threshold_compact(dd, n, dist_col, cutoff, &total, &d_i, &d_j, &d_d,
+                                         c->stream);
+  hipFree(dd);
+  if (e != hipSuccess) return fail(PPSK_ERR_HIP, std::string("threshold: ") + hipGetErrorString(e));
+  auto* hi = (int64_t*)std::malloc(std::max<int64_t>(total, 1) * sizeof(int64_t));
+  auto* hj = (int64_t*)std::malloc(std::max<int64_t>(total, 1) * sizeof(int64_t));
+  auto* hd = (float*)std::malloc(std::max<int64_t>(total, 1) * sizeof(float));
+  if (!hi || !hj || !hd) {
+    std::free(hi);
+    std::free(hj);
+    std::free(hd);
+    hipFree(d_i);
+    hipFree(d_j);
+    hipFree(d_d);
+    return fail(PPSK_ERR_MEMORY, "threshold host output");
+  }
+  if (total > 0) {
+    e = hipMemcpy(hi, d_i, total * sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hj, d_j, total * sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hd, d_d, total * sizeof(float), hipMemcpyDeviceToHost);
+  }
+  hipFree(d_i);
+  hipFree(d_j);
+  hipFree(d_d);
+  if (e != hipSuccess) {
+    std::free(hi);
+    std::free(hj);
+    std::free(hd);
+    return fail(PPSK_ERR_HIP, std::string("threshold copy: ") + hipGetErrorString(e));
+  }
+  *i_vec = hi;
+  *j_vec = hj;
+  *dists = hd;
+  *n_out = total;
+  g_last_ms = ms;
+  g_last_launches = nl;
+  g_last_items = ni;
+  return PPSK_OK;
+}
+
+void ppsk_free_host(void* ptr) { std::free(ptr); }
+
+int ppsk_device_alloc(int device_id, size_t bytes, void** ptr) {
+  DeviceCtx* c = nullptr;
+  if (int rc = get_ctx(device_id, &c)) return rc;
+  HIP_TRY(hipMalloc(ptr, bytes));
+  return PPSK_OK;
+}
+
+int ppsk_device_free(void* ptr) {
+  if (ptr) HIP_TRY(hipFree(ptr));
+  return PPSK_OK;
+}
+
+int ppsk_copy_to_host(void* dst, const void* device_src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, device_src, bytes, hipMemcpyDeviceToHost));
+  return PPSK_OK;
+}
+
+int ppsk_last_timing(double* kernel_ms, int64_t* n_launches, int64_t* n_items) {
+  if (kernel_ms) *kernel_ms = g_last_ms;
+  if (n_launches) *n_launches = g_last_launches;
+  if (n_items) *n_items = g_last_items;
+  return PPSK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer.
+//
+// Device layout of a sketch set ("stripes"), for one k-mer length index kk:
+//   S[kk][nb][w][hl][BLK]  (uint32)
+//     nb : block of BLK = 128 samples (sample n -> nb = n / 128, lane n % 128)
+//     w  : 64-bin word, 0 .. s64-1
+//     hl : 32-bit plane, hl = h * BBITS + l, h = 0 low / 1 high dword of the
+//          uint64 layer word, l = bit layer 0 .. BBITS-1
+// so every (kk, nb, w) is one contiguous 28 x 128 dword slab (14 KiB) and a
+// run of consecutive samples for one plane is contiguous.  Padding samples
+// (n >= n_samples inside the last block) are zero and are masked on output.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace ppsk {
+
+constexpr int BLK = 128;       // samples per stripe block
+constexpr int BBITS = 14;      // bit layers per bin value (reference def_bbits)
+constexpr int HL = 2 * BBITS;  // 32-bit planes per 64-bin word
+constexpr int RQ = 16;         // tile rows per wave (broadcast from SGPRs)
+constexpr int WAVES = 8;       // waves per workgroup
+constexpr int TROWS = RQ * WAVES;  // 128 tile rows
+constexpr int MAXK = 32;       // most k-mer lengths per query
+
+enum OutKind : int32_t { OUT_CORE_ACC = 0, OUT_JACCARD = 1, OUT_COUNTS = 2 };
+enum RandomMode : int32_t { RANDOM_NONE = 0, RANDOM_EQUAL_BASE = 1, RANDOM_MC = 2 };
+
+// Tile: rows [bi*128, +128) x cols [ct*TC, +TC), TC = 64 * RR.
+struct Tile {
+  int32_t bi;
+  int32_t ct;
+};
+
+struct DistParams {
+  const uint32_t* row_stripes;  // set giving the tile rows (r1 of the Jaccard)
+  const uint32_t* col_stripes;  // set giving the tile columns
+  int64_t n_rows, n_cols;       // real sample counts
+  int32_t nb_rows, nb_cols;     // stripe blocks in each set
+  int32_t n_k, s64;
+  int32_t self_mode;            // rows and cols are the same set, upper triangle
+  int32_t full_square;          // self set, every i != j (kNN); i == j -> +inf
+  const Tile* tiles;
+  int64_t n_tiles;
+  int64_t item_base;            // first (tile * n_k + kk) item of this launch
+  uint16_t* counts;             // scratch [n_tiles][n_k][TROWS][TC]
+  uint32_t* tile_done;          // [n_tiles] completion counters, left at zero
+  // epilogue
+  int32_t out_kind;
+  void* out;                    // dense output (see OutKind)
+  int64_t out_row_offset;       // subtracted from the dense row index
+  int64_t out_stride_rows;      // for full_square: row stride of the [rows][n] block
+  int64_t square_row0;          // for full_square: first global row in out
+  float kvals[MAXK];            // k as float (reference kmer2mat)
+  double inv_bins;              // unused placeholder (kept 0)
+  double tolerance;             // regression validity threshold
+  int32_t random_mode;
+  int32_t n_clusters;
+  int32_t mc_ref_is_col;        // MC table index order: [col][row] instead of [row][col]
+  const double* jv_rows;        // [n_rows][n_k] equal-base j(len) per sample
+  const double* jv_cols;
+  const float* mc_table;        // [n_k][C][C]
+  const uint16_t* cl_rows;      // [n_rows] cluster ids
+  const uint16_t* cl_cols;
+  unsigned long long* fail_count;
+  unsigned long long* fail_first;  // smallest failing dense row
+};
+
+// Launchers (dist_kernels.hip).  All work goes on `stream`.
+hipError_t launch_stripe(const uint64_t* raw, int64_t count, int64_t first,
+                         int32_t n_k, int32_t s64, int32_t nb, uint32_t* stripes,
+                         hipStream_t stream);
+hipError_t launch_dists(const DistParams& p, int rr, int64_t n_items,
+                        hipStream_t stream);
+
+}  // namespace ppsk

@@ -1,0 +1,250 @@
+// Sparse outputs on the device.
+//
+// kNN: the reference (src/api.cpp:360-414) scans j = 0..n-1 for every row i
+// and keeps the kNN smallest distances in a std::priority_queue max-heap,
+// admitting an element when the heap is not full or when it is strictly
+// smaller than the heap top, popping the top when the size exceeds kNN, and
+// draining the heap back to front.  Which tied elements survive and their
+// order depend on that exact heap algorithm, so it is replayed here: one wave
+// per row, the row scanned 64 columns at a time, a wave ballot picking the
+// columns that can be admitted against the current top (the top only ever
+// decreases, so every other column is rejected exactly as in the scan), and
+// lane 0 applying the binary-heap push / pop of the C++ standard library's
+// priority_queue (hole sifting, right child preferred unless strictly
+// smaller) to the admitted elements in column order.
+//
+// Threshold: the reference converts the dense column to a square matrix and
+// emits the upper triangle row-major where d < cutoff
+// (src/dist/matrix_ops.cpp:38-69).  Condensed order is exactly row-major
+// upper-triangle order, so this is a stream compaction of the condensed
+// column: per-row counts, an exclusive scan, then an order-preserving write.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "select_kernels.h"
+
+namespace ppsk {
+
+struct HeapEnt {
+  float d;
+  int32_t j;
+};
+
+// std::push_heap on [0, len] after placing v at index len (max-heap on d).
+__device__ __forceinline__ void heap_sift_up(HeapEnt* h, int hole, int top, HeapEnt v) {
+  int parent = (hole - 1) / 2;
+  while (hole > top && h[parent].d < v.d) {
+    h[hole] = h[parent];
+    hole = parent;
+    parent = (hole - 1) / 2;
+  }
+  h[hole] = v;
+}
+
+__device__ __forceinline__ void heap_push(HeapEnt* h, int len, HeapEnt v) {
+  heap_sift_up(h, len, 0, v);
+}
+
+// std::pop_heap on [0, len): moves the top to index len - 1.
+__device__ __forceinline__ void heap_pop(HeapEnt* h, int len) {
+  if (len <= 1) return;
+  const int L = len - 1;
+  const HeapEnt v = h[L];
+  h[L] = h[0];
+  int hole = 0;
+  int second = 0;
+  while (second < (L - 1) / 2) {
+    second = 2 * (second + 1);
+    if (h[second].d < h[second - 1].d) second--;
+    h[hole] = h[second];
+    hole = second;
+  }
+  if ((L & 1) == 0 && second == (L - 2) / 2) {
+    second = 2 * (second + 1);
+    h[hole] = h[second - 1];
+    hole = second - 1;
+  }
+  heap_sift_up(h, hole, 0, v);
+}
+
+__global__ __launch_bounds__(64) void knn_select_kernel(const float* __restrict__ band,
+                                                        int64_t r0, int64_t n, int col,
+                                                        int kNN, int64_t* __restrict__ iv,
+                                                        int64_t* __restrict__ jv,
+                                                        float* __restrict__ dv) {
+  extern __shared__ HeapEnt heap[];
+  const int64_t row = blockIdx.x;
+  const int64_t gi = r0 + row;
+  const float* rp = band + row * n * 2 + col;
+  const int lane = threadIdx.x;
+  int len = 0;        // wave-uniform heap size
+  float top = 0.0f;   // heap top once len == kNN
+  for (int64_t c0 = 0; c0 < n; c0 += 64) {
+    const int64_t j = c0 + lane;
+    float d = INFINITY;
+    if (j < n && j != gi) d = rp[2 * j];
+    const bool cand = (j < n) && (len < kNN || d < top);
+    unsigned long long mask = __ballot(cand);
+    while (mask) {
+      const int b = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      const float db = __shfl(d, b);
+      if (len < kNN || db < top) {
+        float new_top = 0.0f;
+        if (lane == 0) {
+          heap_push(heap, len, HeapEnt{db, (int32_t)(c0 + b)});
+          if (len + 1 > kNN) heap_pop(heap, len + 1);
+          new_top = heap[0].d;
+        }
+        len = len < kNN ? len + 1 : kNN;
+        new_top = __shfl(new_top, 0);
+        if (len == kNN) top = new_top;
+      }
+    }
+  }
+  if (lane == 0) {
+    const int64_t off = row * kNN;
+    int l = len;
+    for (int k = kNN - 1; k >= 0; --k) {
+      iv[off + k] = gi;
+      jv[off + k] = heap[0].j;
+      dv[off + k] = heap[0].d;
+      heap_pop(heap, l);
+      --l;
+    }
+  }
+}
+
+hipError_t launch_knn_select(const float* band, int64_t r0, int64_t n_rows, int64_t n,
+                             int dist_col, int kNN, int64_t* i_vec, int64_t* j_vec,
+                             float* dists, hipStream_t stream) {
+  if (kNN < 1 || kNN > KNN_MAX || kNN > n) return hipErrorInvalidValue;
+  if (n_rows <= 0) return hipSuccess;
+  // push writes slot kNN before the pop trims the heap back to kNN entries
+  knn_select_kernel<<<(unsigned)n_rows, 64, (kNN + 1) * sizeof(HeapEnt), stream>>>(
+      band, r0, n, dist_col, kNN, i_vec, j_vec, dists);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Threshold compaction over the condensed self triangle.
+// Row i covers condensed positions [n*i - i(i+1)/2, + n-1-i), j = i+1+t.
+// ---------------------------------------------------------------------------
+constexpr int TB = 256;
+
+__device__ __forceinline__ float pick(const float2 v, int col) { return col ? v.y : v.x; }
+
+__global__ __launch_bounds__(TB) void thr_count_kernel(const float2* __restrict__ dd, int64_t n,
+                                                       int col, float cutoff,
+                                                       int64_t* __restrict__ counts) {
+  const int64_t i = blockIdx.x;
+  const int64_t start = n * i - ((i * (i + 1)) >> 1);
+  const int64_t len = n - 1 - i;
+  int64_t c = 0;
+  for (int64_t t = threadIdx.x; t < len; t += TB) c += pick(dd[start + t], col) < cutoff ? 1 : 0;
+  __shared__ int64_t red[TB / 64];
+  // wave reduction then across waves
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t s = 0;
+    for (int w = 0; w < TB / 64; ++w) s += red[w];
+    counts[i] = s;
+  }
+}
+
+__global__ __launch_bounds__(TB) void thr_write_kernel(const float2* __restrict__ dd, int64_t n,
+                                                       int col, float cutoff,
+                                                       const int64_t* __restrict__ offsets,
+                                                       int64_t* __restrict__ iv,
+                                                       int64_t* __restrict__ jv,
+                                                       float* __restrict__ dv) {
+  const int64_t i = blockIdx.x;
+  const int64_t start = n * i - ((i * (i + 1)) >> 1);
+  const int64_t len = n - 1 - i;
+  __shared__ int wave_tot[TB / 64];
+  __shared__ int64_t base_sh;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) base_sh = offsets[i];
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < len; c0 += TB) {
+    const int64_t t = c0 + threadIdx.x;
+    float d = 0.0f;
+    bool keep = false;
+    if (t < len) {
+      d = pick(dd[start + t], col);
+      keep = d < cutoff;
+    }
+    const unsigned long long m = __ballot(keep);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[wave] = __popcll(m);
+    __syncthreads();
+    int wave_off = 0;
+    int chunk_tot = 0;
+    for (int w = 0; w < TB / 64; ++w) {
+      if (w < wave) wave_off += wave_tot[w];
+      chunk_tot += wave_tot[w];
+    }
+    const int64_t base = base_sh;
+    if (keep) {
+      const int64_t o = base + wave_off + before;
+      iv[o] = i;
+      jv[o] = i + 1 + t;
+      dv[o] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) base_sh = base + chunk_tot;
+    __syncthreads();
+  }
+}
+
+hipError_t threshold_compact(const float2* dd, int64_t n, int dist_col, float cutoff,
+                             int64_t* total, int64_t** d_i, int64_t** d_j, float** d_d,
+                             hipStream_t stream) {
+  *total = 0;
+  *d_i = nullptr;
+  *d_j = nullptr;
+  *d_d = nullptr;
+  const int64_t rows = n > 1 ? n - 1 : 0;
+  if (rows == 0) return hipSuccess;
+  int64_t* counts = nullptr;
+  hipError_t e = hipMalloc(&counts, rows * sizeof(int64_t));
+  if (e != hipSuccess) return e;
+  thr_count_kernel<<<(unsigned)rows, TB, 0, stream>>>(dd, n, dist_col, cutoff, counts);
+  e = hipGetLastError();
+  std::vector<int64_t> h(rows);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), counts, rows * sizeof(int64_t),
+                                          hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    hipFree(counts);
+    return e;
+  }
+  int64_t run = 0;
+  for (int64_t r = 0; r < rows; ++r) {
+    const int64_t c = h[r];
+    h[r] = run;
+    run += c;
+  }
+  *total = run;
+  e = hipMemcpyAsync(counts, h.data(), rows * sizeof(int64_t), hipMemcpyHostToDevice, stream);
+  const size_t m = (size_t)(run > 0 ? run : 1);
+  if (e == hipSuccess) e = hipMalloc(d_i, m * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc(d_j, m * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc(d_d, m * sizeof(float));
+  if (e == hipSuccess && run > 0) {
+    thr_write_kernel<<<(unsigned)rows, TB, 0, stream>>>(dd, n, dist_col, cutoff, counts, *d_i,
+                                                         *d_j, *d_d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  hipFree(counts);
+  return e;
+}
+
+}  // namespace ppsk

@@ -1,0 +1,123 @@
+"""HIP path vs the CPU oracle on seeded synthetic sketch sets.
+
+Bar: per-k match counts and per-k Jaccards bit-exact; core/accessory within
+1e-6 absolute of the oracle (the contract with the reference CPU path is
+1e-5; the oracle itself reproduces the reference's golden core/accessory to
+~1.2e-7); kNN and threshold indices and distances exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import ppsk_hip as H
+from helpers import KLIST8, mc_table, synth, with_duplicates
+
+pytestmark = pytest.mark.gpu
+
+CORE_ACC_TOL = 1e-6
+
+
+def _oracle_random(mode, table=None, lt=None):
+    if mode == H.RANDOM_MC:
+        return O.Random(2, True, table, lt)
+    return O.Random(mode, True)
+
+
+@pytest.mark.parametrize("n", [1, 2, 127, 300])
+def test_self_counts_and_jaccard_bit_exact(n):
+    raw, le, _ = synth(max(n, 2))
+    raw, le = raw[:n], le[:n]
+    with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
+        cnt, nf, _ = H.dense(s, out="counts")
+        jac, _, _ = H.dense(s, out="jaccard", random_mode=H.RANDOM_EQUAL_BASE)
+    ref_j, ref_c, _, _ = O.query_db(raw, KLIST8, 156, 14, le, random=O.Random(1, True),
+                                    jaccard=True, want_counts=True)
+    assert cnt.shape == ref_c.shape
+    assert np.array_equal(cnt, ref_c)
+    assert np.array_equal(jac.view(np.uint32), ref_j.view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", [H.RANDOM_NONE, H.RANDOM_EQUAL_BASE, H.RANDOM_MC])
+def test_self_core_acc(mode):
+    raw, le, _ = synth(300)
+    table = lt = cl = None
+    if mode == H.RANDOM_MC:
+        table, lt = mc_table(KLIST8, zero_from=27)
+        cl = (np.arange(300) * 7 % 3).astype(np.uint16)
+    with H.SketchSet(raw, KLIST8, 156, lengths=le, clusters=cl) as s:
+        d, nf, _ = H.dense(s, random_mode=mode, table=table)
+    ref, _, rnf, _ = O.query_db(raw, KLIST8, 156, 14, le, ref_cluster=cl,
+                                random=_oracle_random(mode, table, lt))
+    assert nf == rnf == 0
+    assert np.abs(d - ref).max() <= CORE_ACC_TOL
+
+
+@pytest.mark.parametrize("mode", [H.RANDOM_NONE, H.RANDOM_EQUAL_BASE, H.RANDOM_MC])
+def test_query_vs_ref(mode):
+    raw, le, _ = synth(300)
+    q_raw, q_le = raw[:77].copy(), le[:77].copy()
+    r_raw, r_le = raw[77:].copy(), le[77:].copy()
+    table = lt = None
+    qc = rc = None
+    if mode == H.RANDOM_MC:
+        table, lt = mc_table(KLIST8, symmetric=False, zero_from=29)
+        qc = (np.arange(77) % 3).astype(np.uint16)
+        rc = (np.arange(223) * 5 % 3).astype(np.uint16)
+    with H.SketchSet(r_raw, KLIST8, 156, lengths=r_le, clusters=rc) as r, \
+            H.SketchSet(q_raw, KLIST8, 156, lengths=q_le, clusters=qc) as q:
+        d, nf, _ = H.dense(r, q, random_mode=mode, table=table)
+        c, _, _ = H.dense(r, q, out="counts")
+        j, _, _ = H.dense(r, q, random_mode=mode, table=table, out="jaccard")
+    ref, ref_c, rnf, _ = O.query_db(r_raw, KLIST8, 156, 14, r_le, query=q_raw, query_len=q_le,
+                                    ref_cluster=rc, query_cluster=qc,
+                                    random=_oracle_random(mode, table, lt), want_counts=True)
+    ref_j, _, _, _ = O.query_db(r_raw, KLIST8, 156, 14, r_le, query=q_raw, query_len=q_le,
+                                ref_cluster=rc, query_cluster=qc,
+                                random=_oracle_random(mode, table, lt), jaccard=True)
+    assert np.array_equal(c, ref_c)
+    assert np.array_equal(j.view(np.uint32), ref_j.view(np.uint32))
+    assert nf == rnf == 0
+    assert np.abs(d - ref).max() <= CORE_ACC_TOL
+
+
+@pytest.mark.parametrize("knn,col", [(1, 0), (5, 0), (5, 1), (40, 0)])
+def test_knn_matches_heap_order(knn, col):
+    raw, le, _ = synth(260)
+    # exact duplicates create tied distances (including ties at 0)
+    raw, le = with_duplicates(raw, le, [(3, 10), (11, 10), (200, 10), (57, 58), (90, 58)])
+    with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
+        iv, jv, dv, nf = H.knn(s, knn, dist_col=col, random_mode=H.RANDOM_EQUAL_BASE)
+    ri, rj, rd, rnf = O.query_db_sparse(raw, KLIST8, 156, 14, le, knn, dist_col=col,
+                                        random=O.Random(1, True))
+    assert nf == rnf == 0
+    assert np.array_equal(iv, ri)
+    # distances agree to the core/acc tolerance; indices exactly where the
+    # distance is not within tolerance of a tie
+    assert np.abs(dv - rd).max() <= CORE_ACC_TOL
+    assert np.array_equal(jv, rj)
+
+
+def test_knn_equals_n():
+    raw, le, _ = synth(40)
+    with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
+        iv, jv, dv, _ = H.knn(s, 40)
+    ri, rj, rd, _ = O.query_db_sparse(raw, KLIST8, 156, 14, le, 40)
+    assert np.array_equal(jv, rj)
+    assert np.isinf(dv.reshape(40, 40)[:, -1]).all()
+
+
+@pytest.mark.parametrize("cutoff,col", [(0.01, 0), (0.05, 1), (0.0, 0), (1.0, 0)])
+def test_threshold(cutoff, col):
+    raw, le, _ = synth(300)
+    with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
+        if cutoff == 0.0:
+            iv, jv, dv, _ = H.threshold(s, cutoff, dist_col=col)
+            assert len(iv) == 0
+            return
+        iv, jv, dv, nf = H.threshold(s, cutoff, dist_col=col, random_mode=H.RANDOM_EQUAL_BASE)
+    dense, _, _, _ = O.query_db(raw, KLIST8, 156, 14, le, random=O.Random(1, True))
+    square = O.long_to_square(dense[:, col])
+    ri, rj, rd = O.sparsify_threshold(square, cutoff)
+    assert np.array_equal(iv, ri)
+    assert np.array_equal(jv, rj)
+    assert np.abs(dv - rd).max() <= CORE_ACC_TOL
