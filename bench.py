@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Benchmark of the core/accessory distance path on one MI355X.
+
+Workload (one step): all-vs-all core/accessory distances of 8,192 synthetic
+sketches, 8 k-mer lengths (k = 15, 17, ..., 29), 10,000 bins (156 words x 14
+bit layers per k), equal-base random-match correction, dense FP32 output
+(33,550,336 pairs) written to HBM.  Sketches are generated with a seeded
+tree-evolution model and resident in HBM before timing starts.
+
+Prints one JSON line:
+  value      distances (pairs) per second over the timed steps
+  roofline   int32 VALU issue bound of the distance-tile kernel: algorithmic
+             ops per pair = sum_k s64 * (2 * bbits + 2) = 37,440, divided by
+             the kernel's HIP-event duration on its launch stream
+  cpu_baseline  the CPU oracle (C++/OpenMP restatement of the reference CPU
+             path) on a bounded sample of the same workload, on this host
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pp-sketchlib_amd"))
+
+N_SAMPLES = 8192
+KLIST = list(range(15, 30, 2))
+S64 = 156
+BBITS = 14
+OPS_PER_PAIR = len(KLIST) * S64 * (2 * BBITS + 2)  # 37,440
+# vendor peak: 256 CUs x 64 lanes x 2.4 GHz int32 VALU ops
+PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# published GPU figure for this metric (RTX 2080 Ti, 8 k, 10k bins; README)
+PUBLISHED_GPU = 6.0e6
+METRIC = "distances/sec (8 k, 10k bins) on 1 MI355X; int-VALU %peak; p99 assign ms"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=N_SAMPLES)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-sample", type=int, default=4096,
+                    help="samples of the CPU baseline's all-vs-all sample")
+    return ap.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per launch from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "dist_tiles_config2_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(raw, lengths, n_sample, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    sub = raw[:n_sample]
+    le = lengths[:n_sample]
+    pairs = n_sample * (n_sample - 1) // 2
+    t0 = time.perf_counter()
+    O.query_db(sub, KLIST, S64, BBITS, le, random=O.Random(1, True), num_threads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": pairs / dt,
+        "unit": "distances/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"all-vs-all of the first {n_sample} of the {N_SAMPLES} samples "
+                  f"({pairs} pairs, {dt:.1f} s), same k-mer lengths / sketch size / "
+                  f"random correction",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(the distance path shards by tile blocks, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import ppsk_hip as H
+
+    t0 = time.time()
+    raw, lengths, _ = H.synth(a.n, KLIST, S64, seed=1)
+    print(f"# generated {a.n} sketches in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    H.device_init(0)
+    ref = H.SketchSet(raw, KLIST, S64, lengths=lengths)
+    pairs = a.n * (a.n - 1) // 2
+    dout = H.device_alloc(pairs * 8)
+    for _ in range(a.warmup):
+        H.dense_device(ref, random_mode=H.RANDOM_EQUAL_BASE, dptr=dout)
+    kernel_ms = []
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        _, nf = H.dense_device(ref, random_mode=H.RANDOM_EQUAL_BASE, dptr=dout)
+        ms, nl, _ = H.last_timing()
+        kernel_ms.append(ms)
+        launches += nl
+        if nf:
+            raise RuntimeError(f"{nf} pairs failed the k-mer regression")
+    wall = time.perf_counter() - t0
+    print(f"# {a.steps} steps in {wall:.3f}s", file=sys.stderr, flush=True)
+
+    value = pairs * a.steps / wall
+    per_launch_ms = sum(kernel_ms) / launches
+    pairs_per_launch = pairs * a.steps / launches
+    achieved = OPS_PER_PAIR * pairs_per_launch / (per_launch_ms * 1e-3) / 1e12
+    roofline = {
+        "bound": "valu",
+        "achieved": round(achieved, 3),
+        "peak": round(PEAK_TOPS, 3),
+        "unit": "TOP/s",
+        "frac": round(achieved / PEAK_TOPS, 4),
+        "traffic": traffic_from_profile(),
+        "kernel": "dist_tiles",
+        "kernel_ms": round(per_launch_ms, 3),
+        "ops_per_pair": OPS_PER_PAIR,
+    }
+    cpu = None
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(raw, lengths, min(a.cpu_sample, a.n), threads)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "distances/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(value / PUBLISHED_GPU, 2),
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "all-vs-all core/accessory distances, dense FP32 to HBM",
+            "n_samples": a.n,
+            "pairs": pairs,
+            "kmers": KLIST,
+            "bins": S64 * 64,
+            "bbits": BBITS,
+            "random_correction": "equal-base",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -115,15 +115,17 @@ int ppsk_dense(ppsk_set *ref, ppsk_set *query, int32_t random_mode,
 /*
  * Self kNN, as query_db_sparse: for every sample i the kNN smallest of
  * column dist_col (0 core, 1 accessory) over all j (i == j counts as +inf),
- * ascending per row, i_vec/j_vec/dists of n * kNN entries.  Selection and
+ * ascending per row, i_vec/j_vec/dists of n * kNN entries.  With jaccard
+ * set, the ranked distance is 1 - J at k-mer length index dist_col and the
+ * returned values are J again (src/api.cpp:368-371, 423-429).  Selection and
  * tie order reproduce the reference's std::priority_queue scan over j
- * (src/api.cpp:319-432).  Requires 1 <= kNN <= n.
+ * (src/api.cpp:319-432).  Requires 1 <= kNN <= min(n, 2048).
  * Replaces sparseDists (src/gpu/dist.cu:495-791, src/gpu/gpu.hpp:63-73).
  */
 int ppsk_knn(ppsk_set *ref, int32_t random_mode, int32_t n_clusters,
-             const float *mc_table, double tolerance, int32_t kNN,
-             int32_t dist_col, int64_t *i_vec, int64_t *j_vec, float *dists,
-             uint64_t *n_fail);
+             const float *mc_table, double tolerance, int32_t jaccard,
+             int32_t kNN, int32_t dist_col, int64_t *i_vec, int64_t *j_vec,
+             float *dists, uint64_t *n_fail);
 
 /*
  * Self threshold sparsification, as querySelfSparse with dist_cutoff > 0:
@@ -137,7 +139,34 @@ int ppsk_threshold(ppsk_set *ref, int32_t random_mode, int32_t n_clusters,
                    float cutoff, int64_t **i_vec, int64_t **j_vec,
                    float **dists, int64_t *n_out, uint64_t *n_fail);
 
+/*
+ * One self pass serving all three outputs of a large all-vs-all: the
+ * condensed core/accessory column pair (n(n-1)/2 float2 rows, streamed to the
+ * host buffer dense_out in row blocks while later blocks compute; NULL skips
+ * it), the kNN lists of ppsk_knn (kNN = 0 skips them) and the threshold
+ * lists of ppsk_threshold (cutoff <= 0 skips them), all from one distance
+ * computation kept in device memory.  Results equal the three separate
+ * calls.  Needs a symmetric random-match table (always true except for an
+ * asymmetric Monte Carlo table).  dense_out from ppsk_host_alloc makes the
+ * copies fully asynchronous.  Replaces the reference's self queryDatabase
+ * followed by querySelfSparse over the same sketches
+ * (src/sketchlib_bindings.cpp:89-183, src/gpu/dist.cu:408-489).
+ */
+int ppsk_self_pass(ppsk_set *ref, int32_t random_mode, int32_t n_clusters,
+                   const float *mc_table, double tolerance, float *dense_out,
+                   int32_t kNN, int32_t knn_col, int64_t *knn_i,
+                   int64_t *knn_j, float *knn_d, float cutoff,
+                   int32_t thr_col, int64_t **thr_i, int64_t **thr_j,
+                   float **thr_d, int64_t *thr_n, uint64_t *n_fail);
+
 void ppsk_free_host(void *ptr);
+
+/*
+ * Page-locked host memory (hipHostMalloc) for dense outputs, so that the
+ * row-block copies overlap the distance kernels.
+ */
+int ppsk_host_alloc(size_t bytes, void **ptr);
+int ppsk_host_free(void *ptr);
 
 /* Device memory for out_on_device outputs. */
 int ppsk_device_alloc(int device_id, size_t bytes, void **ptr);

@@ -66,9 +66,35 @@ def build_lib():
     return LIB
 
 
+BENCH_LIB = os.path.join(HERE, "libppsk_bench.so")
+
+
+def build_bench_lib():
+    """Measurement kernels (VALU peak, copy calibration) used by bench.py."""
+    tmp = BENCH_LIB + ".tmp"
+    _run([HIPCC, "-x", "hip"] + HIP_FLAGS + ["-shared", os.path.join(CSRC, "peak_kernels.hip"),
+                                             "-o", tmp])
+    os.replace(tmp, BENCH_LIB)
+    return BENCH_LIB
+
+
 def module_path():
     suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
     return os.path.join(HERE, "pp_sketchlib" + suffix)
+
+
+def _vendor_hdf5():
+    """Copy libhdf5 and its zlib next to the module (RUNPATH $ORIGIN/_hdf5).
+
+    Pointing the module's RUNPATH at the HDF5 installation prefix would also
+    resolve libstdc++ from there, shadowing the one the HIP runtime needs."""
+    import shutil
+    dst = os.path.join(HERE, "_hdf5")
+    os.makedirs(dst, exist_ok=True)
+    for name in ("libhdf5.so.103", "libz.so.1"):
+        src = os.path.realpath(os.path.join(HDF5_PREFIX, "lib", name))
+        shutil.copy2(src, os.path.join(dst, name))
+    return dst
 
 
 def build_module():
@@ -78,15 +104,15 @@ def build_module():
     tmp = out + ".tmp"
     py_inc = sysconfig.get_paths()["include"]
     srcs = [os.path.join(CSRC, "host", f) for f in
-            ("sketch_db.cpp", "random_match.cpp", "api.cpp", "matrix_ops.cpp", "bindings.cpp")]
+            ("sketch_db.cpp", "random_match.cpp", "api.cpp", "bindings.cpp")]
     cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
            "-ffp-contract=off", "-fvisibility=hidden",
            "-I", INCLUDE, "-I", CSRC, "-I", pybind11.get_include(), "-I", py_inc,
            "-I", os.path.join(HDF5_PREFIX, "include"),
            "-DPPSK_VERSION_INFO=\"2.1.5\""] + srcs + [
            "-L", HERE, "-lppsk",
-           "-L", os.path.join(HDF5_PREFIX, "lib"), "-l:libhdf5.so.103",
-           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(HDF5_PREFIX, "lib"),
+           os.path.join(_vendor_hdf5(), "libhdf5.so.103"),
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/_hdf5",
            "-o", tmp]
     _run(cmd)
     os.replace(tmp, out)
@@ -95,6 +121,7 @@ def build_module():
 
 def build_all(module=True):
     build_lib()
+    build_bench_lib()
     if module and os.path.isdir(os.path.join(CSRC, "host")):
         build_module()
 

@@ -9,14 +9,13 @@
 // per 64-bin word per pair, which is the algorithmic op count.
 //
 // Tiling (GEMM-shaped, integer VALU, no MFMA):
-//  * a workgroup owns a 128 x TC tile of pairs (TC = 64 * RR) for ONE k-mer
-//    length; 8 waves x 16 rows each.
-//  * the 16 row samples of a wave are wave-uniform, so their planes are read
-//    with scalar loads and sit in SGPRs: each v_bitop3 takes its row operand
-//    from an SGPR for free;
-//  * column samples are per lane (RR per lane); their planes are staged per
-//    group of words through LDS by the whole workgroup (coalesced 16-byte
-//    global loads of contiguous stripe slabs) and read back per lane;
+//  * a workgroup owns a (8 RQ) x (64 RR) tile of pairs for ONE k-mer length;
+//    8 waves x RQ rows each, RR columns per lane;
+//  * the RQ row samples of a wave are wave-uniform, so their planes are read
+//    with scalar loads into SGPRs: each v_bitop3 takes its row operand from
+//    an SGPR;
+//  * column planes are staged per group of words straight from global memory
+//    into LDS (global_load_lds_dwordx4, double buffered) and read per lane;
 //  * per-pair counts live in VGPRs for the whole word loop, then go to a
 //    per-tile count scratch;
 //  * the last of the n_k workgroups of a tile to finish (device-scope atomic)
@@ -24,8 +23,8 @@
 //    log-Jaccard-vs-k regression, dense output.
 #include <hip/hip_runtime.h>
 
-#include <cstdint>
 #include <cmath>
+#include <cstdint>
 
 #include "ppsk_internal.h"
 
@@ -73,40 +72,35 @@ hipError_t launch_stripe(const uint64_t* raw, int64_t count, int64_t first,
 }
 
 // ---------------------------------------------------------------------------
-// Distance tiles
+// Helpers
 // ---------------------------------------------------------------------------
+typedef const __attribute__((address_space(4))) uint32_t CU32;
+typedef const __attribute__((address_space(4))) DistParams CParams;
+
 __device__ __forceinline__ uint32_t match_and(uint32_t acc, uint32_t a, uint32_t b) {
   // truth table over (acc, a, b) with src0 = 0xF0, src1 = 0xCC, src2 = 0xAA:
   // acc & ~(a ^ b) = 0x90
   return __builtin_amdgcn_bitop3_b32(acc, a, b, 0x90);
 }
 
-// Words staged through LDS per step: WS * 28 planes * TC dwords = 512 uint4
-// per plane-row group x 7, i.e. 7 16-byte loads per thread per stage.
-template <int RR>
-struct TileCfg {
-  static constexpr int TC = 64 * RR;
-  static constexpr int WS = 8 / RR;
-  static constexpr int STAGE_U4 = WS * HL * TC / 4;  // 3584
-  static constexpr int U4_PER_THREAD = STAGE_U4 / (WAVES * 64);  // 7
-  static_assert(STAGE_U4 % (WAVES * 64) == 0, "stage must split evenly");
-};
+// v_bcnt_u32_b32 with accumulate: cnt + popcount(x) in one VALU op (the
+// compiler otherwise splits paired popcounts into two bcnt and a v_add3).
+__device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t cnt) {
+  uint32_t r;
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(cnt));
+  return r;
+}
 
-// Per-pair epilogue: counts of the n_k lengths -> dense output.
-// Reference semantics (CPU path):
+// Per-pair epilogue pieces.  Reference semantics (CPU path):
 //  J_obs = c / (64 * s64)                        reference.cpp:222-232
 //  J     = max(J_obs - r, 0) * 1 / (1 - r)       dist/dist.hpp:13-24 (double)
 //  dists[kk] = (float) J                         linear_regression.cpp:62-71
 //  window [first J > tol, last J > tol], LS of log J on k, core/acc
 //                                                linear_regression.cpp:16-60
-struct PairOut {
-  float core, acc;
-  bool ok;
-};
-
-__device__ __forceinline__ double random_for(const DistParams& p, int kk, int64_t gi,
-                                             int64_t gj) {
+template <typename P>
+__device__ __forceinline__ double random_for(const P& p, int kk, int64_t gi, int64_t gj) {
   if (p.random_mode == RANDOM_EQUAL_BASE) {
+    // csrs combination (random_match.cpp:333-334) of per-sample terms
     const double j1 = p.jv_rows[gi * p.n_k + kk];
     const double j2 = p.jv_cols[gj * p.n_k + kk];
     double r = 0.0;
@@ -123,139 +117,308 @@ __device__ __forceinline__ double random_for(const DistParams& p, int kk, int64_
   return 0.0;
 }
 
-__device__ __forceinline__ float jaccard_of(uint32_t c, double bins, double r) {
+__device__ __forceinline__ double jaccard_d(uint32_t c, double bins, double r) {
   const double obs = (double)c / bins;
   const double diff = obs > r ? (obs - r) : 0.0;
-  return (float)((diff * 1.0) / (1.0 - r));
+  return (diff * 1.0) / (1.0 - r);
 }
 
-template <int RR>
-__global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
-  using Cfg = TileCfg<RR>;
+// ---------------------------------------------------------------------------
+// Tile shape and LDS staging
+// ---------------------------------------------------------------------------
+// A stage is WS words x 28 planes x TC dwords = 1792 uint4 (28 KiB), double
+// buffered (56 KiB per workgroup, two workgroups per CU).  Stages are copied
+// global -> LDS directly: round i of wave w moves uint4 [i*512 + w*64, +64)
+// of the stage; the last round is a half round.
+template <int RQ, int RR>
+struct TileCfg {
+  static constexpr int TROWS = WAVES * RQ;
+  static constexpr int TC = 64 * RR;
+  static constexpr int WS = 4 / RR;
+  static constexpr int STAGE_U4 = WS * HL * TC / 4;  // 1792
+  static constexpr int ROUNDS = (STAGE_U4 + WAVES * 64 - 1) / (WAVES * 64);  // 4
+  static_assert(STAGE_U4 % 64 == 0, "stage must be whole wave rounds");
+  static_assert(BLK % TROWS == 0, "row tiles must not straddle stripe blocks");
+};
+
+// s_waitcnt vmcnt(n) with expcnt / lgkmcnt left unconstrained (gfx9 encoding)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// LDS byte address of a __shared__ object.
+__device__ __forceinline__ uint32_t lds_addr(const uint4* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+// One 16-byte-per-lane global -> LDS copy: lane L writes LDS bytes
+// [m0 + 16 L, +16).  Written as inline asm so that the compiler's wait-count
+// pass does not treat it as a pending write to every LDS buffer (it cannot
+// tell the two stage buffers apart and would wait for the copy of the next
+// stage before reading the current one); completion is awaited explicitly
+// with vmcnt before the stage barrier.
+__device__ __forceinline__ void dma16(const uint32_t* g, uint32_t lds_byte_addr) {
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
+               :
+               : "s"(lds_byte_addr), "v"(g)
+               : "memory", "m0");
+}
+
+// Workgroup barrier without the release/acquire fences of __syncthreads():
+// those wait for every outstanding vector-memory operation, including the
+// next stage's direct-to-LDS copy that is meant to overlap the matching.
+// Ordering of the stage copies is handled explicitly with vmcnt waits.
+__device__ __forceinline__ void stage_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+// Column operands of one stage: words [w0, w0 + WS) of the tile's TC columns,
+// which may span two stripe blocks (TC = 256) or half of one (TC = 64).
+// Columns past the last stripe block read the last block (masked later).
+template <int RQ, int RR>
+__device__ __forceinline__ void issue_stage(const uint32_t* __restrict__ col_kk, int64_t col0,
+                                            int nb_cols, int s64, int w0, uint4* buf, int wave,
+                                            int lane) {
+  using Cfg = TileCfg<RQ, RR>;
   constexpr int TC = Cfg::TC;
-  constexpr int WS = Cfg::WS;
-  __shared__ uint4 sB4[Cfg::STAGE_U4];
-  __shared__ int s_last;
-  const uint32_t* sB = reinterpret_cast<const uint32_t*>(sB4);
-
-  const int64_t item = p.item_base + blockIdx.x;
-  const int64_t t = item / p.n_k;
-  const int kk = (int)(item - t * p.n_k);
-  const Tile tile = p.tiles[t];
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int s64 = p.s64;
-
-  // row operands: 16 consecutive samples of this wave, wave-uniform address
-  const uint32_t* __restrict__ arow =
-      p.row_stripes + ((int64_t)kk * p.nb_rows + tile.bi) * (int64_t)s64 * (HL * BLK) +
-      wave * RQ;
-  // column slab: stripe block of the column tile and the offset inside it
-  const int64_t col0 = (int64_t)tile.ct * TC;
-  const int cb = (int)(col0 / BLK);
-  const int c_in = (int)(col0 % BLK);
-  const uint32_t* __restrict__ bsrc =
-      p.col_stripes + ((int64_t)kk * p.nb_cols + cb) * (int64_t)s64 * (HL * BLK) + c_in;
-
-  uint32_t cnt[RQ][RR];
 #pragma unroll
-  for (int q = 0; q < RQ; ++q)
-#pragma unroll
-    for (int r = 0; r < RR; ++r) cnt[q][r] = 0;
-
-  // global -> register staging of one stage (WS words of the column slab)
-  uint4 stage[Cfg::U4_PER_THREAD];
-  auto load_stage = [&](int w0) {
-#pragma unroll
-    for (int i = 0; i < Cfg::U4_PER_THREAD; ++i) {
-      const int x = threadIdx.x + i * (WAVES * 64);   // uint4 index in stage
-      const int u = x % (TC / 4);
-      const int hl = (x / (TC / 4)) % HL;
-      const int ws = x / (HL * TC / 4);
-      const int w = w0 + ws;
-      if (w < s64) {
-        stage[i] = *reinterpret_cast<const uint4*>(bsrc + ((int64_t)w * HL + hl) * BLK + u * 4);
-      } else {
-        stage[i] = make_uint4(0, 0, 0, 0);
-      }
+  for (int i = 0; i < Cfg::ROUNDS; ++i) {
+    const int x0 = i * (WAVES * 64) + wave * 64;  // wave-uniform
+    if (x0 < Cfg::STAGE_U4) {
+      const int x = x0 + lane;
+      const int row = x / (TC / 4);  // ws * HL + hl
+      const int64_t col = col0 + (x % (TC / 4)) * 4;
+      int blk = (int)(col / BLK);
+      blk = blk < nb_cols ? blk : nb_cols - 1;
+      const uint32_t* g =
+          col_kk + ((int64_t)blk * s64 + w0) * (HL * BLK) + (int64_t)row * BLK + (col % BLK);
+      dma16(g, lds_addr(buf + x0));
     }
-  };
-  auto store_stage = [&]() {
-#pragma unroll
-    for (int i = 0; i < Cfg::U4_PER_THREAD; ++i) sB4[threadIdx.x + i * (WAVES * 64)] = stage[i];
-  };
+  }
+}
 
-  const int n_stages = (s64 + WS - 1) / WS;
-  load_stage(0);
-  store_stage();
-  __syncthreads();
-  for (int st = 0; st < n_stages; ++st) {
-    const int w0 = st * WS;
-    if (st + 1 < n_stages) load_stage(w0 + WS);
-    const int nw = (s64 - w0) < WS ? (s64 - w0) : WS;
-    for (int ws = 0; ws < nw; ++ws) {
-      const uint32_t* __restrict__ aw = arow + (int64_t)(w0 + ws) * (HL * BLK);
-      const uint32_t* bw = sB + ws * (HL * TC) + lane;
+// Loads still outstanding for a wave after it issued one more stage.
+template <int RQ, int RR>
+__device__ __forceinline__ void wait_stage_but_one(int wave) {
+  using Cfg = TileCfg<RQ, RR>;
+  constexpr int FULL = Cfg::STAGE_U4 / (WAVES * 64);  // rounds every wave issues
+  constexpr int REM_WAVES = (Cfg::STAGE_U4 % (WAVES * 64)) / 64;
+  if (wave < REM_WAVES) {
+    wait_vmcnt<FULL + 1>();
+  } else {
+    wait_vmcnt<FULL>();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Matching, row operands left to the compiler (PB = 0): one plane at a time.
+// ---------------------------------------------------------------------------
+template <int RQ, int RR>
+__device__ __forceinline__ void match_stage_plain(const CU32* arow, int w0, int nw,
+                                                  const uint32_t* sB, int lane,
+                                                  uint32_t (&cnt)[RQ][RR]) {
+  constexpr int TC = TileCfg<RQ, RR>::TC;
+  for (int ws = 0; ws < nw; ++ws) {
+    const CU32* aw = arow + (int64_t)(w0 + ws) * (HL * BLK);
+    const uint32_t* bw = sB + ws * (HL * TC) + lane;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        uint32_t acc[RQ][RR];
+    for (int h = 0; h < 2; ++h) {
+      uint32_t acc[RQ][RR];
 #pragma unroll
-        for (int l = 0; l < BBITS; ++l) {
-          const int hl = h * BBITS + l;
-          uint32_t b[RR];
+      for (int l = 0; l < BBITS; ++l) {
+        const int hl = h * BBITS + l;
+        uint32_t a[RQ], b[RR];
 #pragma unroll
-          for (int r = 0; r < RR; ++r) b[r] = bw[hl * TC + 64 * r];
+        for (int q = 0; q < RQ; ++q) a[q] = aw[hl * BLK + q];
 #pragma unroll
-          for (int q = 0; q < RQ; ++q) {
-            const uint32_t a = aw[hl * BLK + q];
+        for (int r = 0; r < RR; ++r) b[r] = bw[hl * TC + 64 * r];
 #pragma unroll
-            for (int r = 0; r < RR; ++r) {
-              if (l == 0) {
-                acc[q][r] = ~(a ^ b[r]);
-              } else {
-                acc[q][r] = match_and(acc[q][r], a, b[r]);
-              }
-            }
+        for (int q = 0; q < RQ; ++q) {
+#pragma unroll
+          for (int r = 0; r < RR; ++r) {
+            acc[q][r] = (l == 0) ? ~(a[q] ^ b[r]) : match_and(acc[q][r], a[q], b[r]);
           }
         }
+      }
 #pragma unroll
-        for (int q = 0; q < RQ; ++q)
+      for (int q = 0; q < RQ; ++q)
 #pragma unroll
-          for (int r = 0; r < RR; ++r) cnt[q][r] += __builtin_popcount(acc[q][r]);
+        for (int r = 0; r < RR; ++r) cnt[q][r] = bcnt_acc(acc[q][r], cnt[q][r]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Matching with an explicit two-deep operand pipeline (PB = 1 or 2).
+// Scalar loads return out of order, so the only possible wait for them is
+// "all outstanding" (lgkmcnt(0)), which also covers every LDS read; left to
+// itself the compiler issues one plane's s_load and waits for it right away,
+// exposing the full L2 latency every RQ * RR VALU ops.  Here the row
+// operands of PB planes (scalar loads into SGPR tuples) and the matching
+// column operands (ds_read2st64 into VGPR pairs) form one batch: batch b+1
+// is issued before batch b is matched, and a single wait that names the
+// registers it completes follows the matching.  Each round trip therefore
+// overlaps PB * RQ * RR bitop3 of the same wave plus the other waves of the
+// SIMD.  All loads of the pipeline are issued from inline asm, so the
+// compiler's own wait counting never sees (or waits on) them.
+// ---------------------------------------------------------------------------
+template <int RQ>
+struct ATuple;
+template <>
+struct ATuple<16> {
+  typedef uint32_t T __attribute__((ext_vector_type(16)));
+};
+template <>
+struct ATuple<8> {
+  typedef uint32_t T __attribute__((ext_vector_type(8)));
+};
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <int RQ, int RR, int PB>
+struct Batch {
+  static_assert(RR % 2 == 0, "column operands are read in pairs");
+  typename ATuple<RQ>::T a[PB];  // row planes: RQ samples each
+  u32x2 b[PB][RR / 2];           // column planes: RR per lane
+};
+
+template <int RQ, int OFF>
+__device__ __forceinline__ void a_load(const CU32* aw, typename ATuple<RQ>::T& x) {
+  if constexpr (RQ == 16) {
+    asm volatile("s_load_dwordx16 %0, %1, %2" : "=&s"(x) : "s"(aw), "i"(OFF));
+  } else {
+    asm volatile("s_load_dwordx8 %0, %1, %2" : "=&s"(x) : "s"(aw), "i"(OFF));
+  }
+}
+
+// Issue planes P0 .. P0+PB-1 of the word at (aw, bw); P0 = HL addresses the
+// first planes of the next word.  bw is the LDS byte address of plane 0,
+// column `lane` of the word; column r of plane P sits at bw + (P TC + 64 r) 4,
+// i.e. at offset P TC / 64 + r in the 256-byte units of ds_read2st64.
+template <int RQ, int RR, int PB, int P0>
+__device__ __forceinline__ void batch_issue(const CU32* aw, uint32_t bw, Batch<RQ, RR, PB>& x) {
+  constexpr int TC = 64 * RR;
+#pragma unroll
+  for (int pb = 0; pb < PB; ++pb) {
+    if (pb == 0) a_load<RQ, P0 * BLK * 4>(aw, x.a[0]);
+    if (pb == 1) a_load<RQ, (P0 + 1) * BLK * 4>(aw, x.a[PB - 1]);
+  }
+#pragma unroll
+  for (int pb = 0; pb < PB; ++pb) {
+#pragma unroll
+    for (int rp = 0; rp < RR / 2; ++rp) {
+      asm volatile("ds_read2st64_b32 %0, %1 offset0:%2 offset1:%3"
+                   : "=&v"(x.b[pb][rp])
+                   : "v"(bw), "i"((P0 + pb) * TC / 64 + 2 * rp),
+                     "i"((P0 + pb) * TC / 64 + 2 * rp + 1));
+    }
+  }
+}
+
+template <int RQ, int RR, int PB>
+__device__ __forceinline__ void batch_wait(Batch<RQ, RR, PB>& x) {
+  if constexpr (PB == 2 && RR == 4) {
+    asm volatile("s_waitcnt lgkmcnt(0)"
+                 : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0][0]), "+v"(x.b[0][1]),
+                   "+v"(x.b[1][0]), "+v"(x.b[1][1]));
+  } else if constexpr (PB == 2 && RR == 2) {
+    asm volatile("s_waitcnt lgkmcnt(0)"
+                 : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0][0]), "+v"(x.b[1][0]));
+  } else if constexpr (PB == 1 && RR == 4) {
+    asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(x.a[0]), "+v"(x.b[0][0]), "+v"(x.b[0][1]));
+  } else {
+    static_assert(PB == 1 && RR == 2, "unsupported batch shape");
+    asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(x.a[0]), "+v"(x.b[0][0]));
+  }
+}
+
+// Planes P .. P+PB-1 of one word against the lane's RR columns.
+template <int RQ, int RR, int PB, int P>
+__device__ __forceinline__ void match_batch(const Batch<RQ, RR, PB>& x, uint32_t (&acc)[RQ][RR]) {
+#pragma unroll
+  for (int pb = 0; pb < PB; ++pb) {
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+#pragma unroll
+      for (int r = 0; r < RR; ++r) {
+        const uint32_t a = x.a[pb][q];
+        const uint32_t b = x.b[pb][r / 2][r % 2];
+        if ((P + pb) % BBITS == 0) {
+          acc[q][r] = ~(a ^ b);
+        } else {
+          acc[q][r] = match_and(acc[q][r], a, b);
+        }
       }
     }
-    __syncthreads();
-    if (st + 1 < n_stages) {
-      store_stage();
-      __syncthreads();
-    }
   }
+}
 
-  // per-tile count scratch: [tile][kk][row][col]
-  uint16_t* cbase = p.counts + ((t * p.n_k + kk) * TROWS + wave * RQ) * (int64_t)TC + lane;
-#pragma unroll
-  for (int q = 0; q < RQ; ++q)
-#pragma unroll
-    for (int r = 0; r < RR; ++r) cbase[q * TC + 64 * r] = (uint16_t)cnt[q][r];
-
-  // completion: the last workgroup of this tile runs the epilogue
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned prev = atomicAdd(&p.tile_done[t], 1u);
-    s_last = (prev == (unsigned)(p.n_k - 1)) ? 1 : 0;
+// Batches B .. NB-1 of one word; cur holds batch B (loaded and complete).
+// The last batch prefetches the first batch of the next word (if any) into
+// the registers the next word starts from.
+template <int RQ, int RR, int PB, int B>
+__device__ __forceinline__ void match_batches(const CU32* aw, uint32_t bw, bool next_word,
+                                              Batch<RQ, RR, PB>& cur, Batch<RQ, RR, PB>& nxt,
+                                              uint32_t (&acc)[RQ][RR], uint32_t (&cnt)[RQ][RR]) {
+  constexpr int NB = HL / PB;
+  if constexpr (B + 1 < NB) {
+    batch_issue<RQ, RR, PB, (B + 1) * PB>(aw, bw, nxt);
+  } else {
+    if (next_word) batch_issue<RQ, RR, PB, HL>(aw, bw, nxt);  // word w + 1
   }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  if (threadIdx.x == 0) p.tile_done[t] = 0;  // ready for the next launch
+  match_batch<RQ, RR, PB, B * PB>(cur, acc);
+  if constexpr (((B + 1) * PB) % BBITS == 0) {
+    // end of a 32-bit half: fold its matches into the counts
+#pragma unroll
+    for (int q = 0; q < RQ; ++q)
+#pragma unroll
+      for (int r = 0; r < RR; ++r) cnt[q][r] = bcnt_acc(acc[q][r], cnt[q][r]);
+  }
+  if constexpr (B + 1 < NB) {
+    batch_wait<RQ, RR, PB>(nxt);
+    match_batches<RQ, RR, PB, B + 1>(aw, bw, next_word, nxt, cur, acc, cnt);
+  }
+}
 
-  const double bins = 64.0 * (double)s64;
-  const int64_t gi0 = (int64_t)tile.bi * BLK + wave * RQ;
+template <int RQ, int RR, int PB>
+__device__ __forceinline__ void match_stage_pipe(const CU32* arow, int w0, int nw,
+                                                 const uint4* sbuf, int lane,
+                                                 uint32_t (&cnt)[RQ][RR]) {
+  constexpr int TC = TileCfg<RQ, RR>::TC;
+  static_assert((HL / PB) % 2 == 0, "an even batch count keeps the register sets aligned");
+  Batch<RQ, RR, PB> c, n;
+  const CU32* aw = arow + (int64_t)w0 * (HL * BLK);
+  uint32_t bw = lds_addr(sbuf) + lane * 4;
+  batch_issue<RQ, RR, PB, 0>(aw, bw, c);
+  for (int ws = 0; ws < nw; ++ws) {
+    batch_wait<RQ, RR, PB>(c);
+    uint32_t acc[RQ][RR];
+    // batches alternate (c, n); the last one (odd) leaves the next word's
+    // first batch in c
+    match_batches<RQ, RR, PB, 0>(aw, bw, ws + 1 < nw, c, n, acc, cnt);
+    aw += HL * BLK;
+    bw += HL * TC * 4;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused epilogue of one tile (run by the last of its n_k workgroups):
+// per-k counts -> random-corrected Jaccard -> dense output.  Out of line so
+// that none of its state occupies registers during the matching loop.
+// ---------------------------------------------------------------------------
+template <int RQ, int RR>
+__device__ __noinline__ void tile_epilogue(CParams* pp, int64_t t, Tile tile, int wave,
+                                           int lane) {
+  CParams& p = *pp;
+  using Cfg = TileCfg<RQ, RR>;
+  constexpr int TC = Cfg::TC;
+  constexpr int TROWS = Cfg::TROWS;
+  const double bins = 64.0 * (double)p.s64;
+  const int64_t gi0 = (int64_t)tile.bi * TROWS + wave * RQ;
+  const int64_t col0 = (int64_t)tile.ct * TC;
   const uint16_t* tbase = p.counts + (t * p.n_k * TROWS + wave * RQ) * (int64_t)TC + lane;
+  const int64_t kstride = (int64_t)TROWS * TC;
   for (int q = 0; q < RQ; ++q) {
     const int64_t gi = gi0 + q;
-#pragma unroll
     for (int r = 0; r < RR; ++r) {
       const int64_t gj = col0 + lane + 64 * r;
       bool valid;
@@ -277,14 +440,24 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
       const uint16_t* cp = tbase + q * TC + 64 * r;
       if (p.out_kind == OUT_COUNTS) {
         uint16_t* o = reinterpret_cast<uint16_t*>(p.out) + pos * p.n_k;
-        for (int k = 0; k < p.n_k; ++k) o[k] = cp[(int64_t)k * TROWS * TC];
+        for (int k = 0; k < p.n_k; ++k) o[k] = cp[k * kstride];
         continue;
       }
       if (p.out_kind == OUT_JACCARD) {
         float* o = reinterpret_cast<float*>(p.out) + pos * p.n_k;
         for (int k = 0; k < p.n_k; ++k) {
-          o[k] = jaccard_of(cp[(int64_t)k * TROWS * TC], bins, random_for(p, k, gi, gj));
+          o[k] = (float)jaccard_d(cp[k * kstride], bins, random_for(p, k, gi, gj));
         }
+        continue;
+      }
+      if (p.out_kind == OUT_ONE_MINUS_J) {
+        // kNN on Jaccards: (float)(1 - J) of the first two k-mer lengths
+        // (src/api.cpp:368-371 narrows 1.0f - double J to float)
+        float v[2] = {INFINITY, INFINITY};
+        for (int k = 0; k < 2 && k < p.n_k; ++k) {
+          v[k] = (float)(1.0 - jaccard_d(cp[k * kstride], bins, random_for(p, k, gi, gj)));
+        }
+        reinterpret_cast<float2*>(p.out)[pos] = make_float2(v[0], v[1]);
         continue;
       }
       // core/acc: streaming window fit.  Points before the first valid one are
@@ -293,7 +466,7 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
       int first = -1, last = -1;
       double sy = 0.0, sxy = 0.0, py = 0.0, pxy = 0.0;
       for (int k = 0; k < p.n_k; ++k) {
-        const float jf = jaccard_of(cp[(int64_t)k * TROWS * TC], bins, random_for(p, k, gi, gj));
+        const float jf = (float)jaccard_d(cp[k * kstride], bins, random_for(p, k, gi, gj));
         const bool ok = (double)jf > p.tolerance;
         if (!ok && first < 0) continue;
         const double x = (double)p.kvals[k];
@@ -337,18 +510,143 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
         if (slope < 0) core = (float)(1.0 - exp(slope));
         if (intercept < 0) accd = (float)(1.0 - exp(intercept));
       }
-      float2* o = reinterpret_cast<float2*>(p.out) + pos;
-      *o = make_float2(core, accd);
+      reinterpret_cast<float2*>(p.out)[pos] = make_float2(core, accd);
     }
   }
 }
 
-hipError_t launch_dists(const DistParams& p, int rr, int64_t n_items, hipStream_t stream) {
+// ---------------------------------------------------------------------------
+// The tile kernel: one (tile, k-mer length) item per workgroup.
+// ---------------------------------------------------------------------------
+template <int RQ, int RR, int PB>
+__global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
+  using Cfg = TileCfg<RQ, RR>;
+  constexpr int TC = Cfg::TC;
+  constexpr int WS = Cfg::WS;
+  constexpr int TROWS = Cfg::TROWS;
+  __shared__ uint4 sB0[Cfg::STAGE_U4];
+  __shared__ uint4 sB1[Cfg::STAGE_U4];
+  __shared__ int s_last;
+
+  const int64_t item = p.item_base + blockIdx.x;
+  const int64_t t = item / p.n_k;
+  const int kk = (int)(item - t * p.n_k);
+  Tile tile = p.tiles[t];
+  // wave-uniform values (kept in SGPRs)
+  tile.bi = __builtin_amdgcn_readfirstlane(tile.bi);
+  tile.ct = __builtin_amdgcn_readfirstlane(tile.ct);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int s64 = p.s64;
+
+  // row operands: RQ consecutive samples of this wave, wave-uniform address,
+  // read through the constant address space so they are scalar loads into
+  // SGPRs (the stripes are never written while a distance kernel runs)
+  const int64_t row0 = (int64_t)tile.bi * TROWS;
+  const CU32* arow = (CU32*)(p.row_stripes +
+                             ((int64_t)kk * p.nb_rows + row0 / BLK) * (int64_t)s64 * (HL * BLK) +
+                             row0 % BLK + wave * RQ);
+  // column slabs of this k-mer length.  The last stage may run past word
+  // s64 - 1: the stripe buffer carries zero padding at its end, and those
+  // words are never computed on.
+  const int64_t col0 = (int64_t)tile.ct * TC;
+  const uint32_t* __restrict__ col_kk =
+      p.col_stripes + (int64_t)kk * p.nb_cols * (int64_t)s64 * (HL * BLK);
+
+  uint32_t cnt[RQ][RR];
+#pragma unroll
+  for (int q = 0; q < RQ; ++q)
+#pragma unroll
+    for (int r = 0; r < RR; ++r) cnt[q][r] = 0;
+
+  const int n_stages = (s64 + WS - 1) / WS;
+  issue_stage<RQ, RR>(col_kk, col0, p.nb_cols, s64, 0, sB0, wave, lane);
+  // stages alternate between the two buffers; each pass of this loop runs an
+  // even stage from sB0 and the following odd stage from sB1
+  for (int st = 0; st < n_stages; st += 2) {
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int s = st + half;
+      if (s < n_stages) {
+        uint4* cur = half ? sB1 : sB0;
+        uint4* nxt = half ? sB0 : sB1;
+        const int w0 = s * WS;
+        if (s + 1 < n_stages) {
+          issue_stage<RQ, RR>(col_kk, col0, p.nb_cols, s64, w0 + WS, nxt, wave, lane);
+          wait_stage_but_one<RQ, RR>(wave);
+        } else {
+          wait_vmcnt<0>();
+        }
+        stage_barrier();
+        const int nw = (s64 - w0) < WS ? (s64 - w0) : WS;
+        const uint32_t* sB = reinterpret_cast<const uint32_t*>(cur);
+        if constexpr (PB == 0) {
+          match_stage_plain<RQ, RR>(arow, w0, nw, sB, lane, cnt);
+        } else {
+          match_stage_pipe<RQ, RR, PB>(arow, w0, nw, cur, lane, cnt);
+        }
+        stage_barrier();
+      }
+    }
+  }
+
+  // per-tile count scratch: [tile][kk][row][col]
+  uint16_t* cbase = p.counts + ((t * p.n_k + kk) * TROWS + wave * RQ) * (int64_t)TC + lane;
+#pragma unroll
+  for (int q = 0; q < RQ; ++q)
+#pragma unroll
+    for (int r = 0; r < RR; ++r) cbase[q * TC + 64 * r] = (uint16_t)cnt[q][r];
+
+  // completion: the last workgroup of this tile runs the epilogue
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned prev = atomicAdd(&p.tile_done[t], 1u);
+    s_last = (prev == (unsigned)(p.n_k - 1)) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  if (threadIdx.x == 0) p.tile_done[t] = 0;  // ready for the next launch
+  // the parameters are the kernel's only argument: hand the epilogue the
+  // kernarg segment itself rather than a copy
+  tile_epilogue<RQ, RR>((CParams*)__builtin_amdgcn_kernarg_segment_ptr(), t, tile, wave, lane);
+}
+
+bool variant_valid(int v) { return v >= 1 && v <= 5; }
+
+int variant_tile_rows(int v) { return (v == 2 || v == 5) ? 64 : 128; }
+
+int variant_tile_cols(int v) {
+  switch (v) {
+    case 1:
+      return 64;
+    case 2:
+      return 256;
+    default:
+      return 128;
+  }
+}
+
+hipError_t launch_dists(const DistParams& p, int variant, int64_t n_items, hipStream_t stream) {
   if (n_items <= 0) return hipSuccess;
-  if (rr == 2) {
-    dist_tiles<2><<<(unsigned)n_items, WAVES * 64, 0, stream>>>(p);
-  } else {
-    dist_tiles<1><<<(unsigned)n_items, WAVES * 64, 0, stream>>>(p);
+  const unsigned g = (unsigned)n_items;
+  switch (variant) {
+    case 1:
+      dist_tiles<16, 1, 0><<<g, WAVES * 64, 0, stream>>>(p);
+      break;
+    case 3:
+      dist_tiles<16, 2, 0><<<g, WAVES * 64, 0, stream>>>(p);
+      break;
+    case 4:
+      dist_tiles<16, 2, 1><<<g, WAVES * 64, 0, stream>>>(p);
+      break;
+    case 5:
+      dist_tiles<8, 2, 2><<<g, WAVES * 64, 0, stream>>>(p);
+      break;
+    default:
+      dist_tiles<8, 4, 2><<<g, WAVES * 64, 0, stream>>>(p);
+      break;
   }
   return hipGetLastError();
 }

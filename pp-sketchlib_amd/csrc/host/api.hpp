@@ -1,0 +1,72 @@
+// Host orchestration of the distance path on top of the C ABI (libppsk):
+// the roles of query_db / query_db_sparse / query_db_cuda /
+// query_db_sparse_cuda (src/api.cpp:140-432, src/gpu/gpu_api.cpp:352-531).
+// Every distance is computed by the HIP kernels; there is no CPU path.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "random_match.hpp"
+#include "sketch_db.hpp"
+
+namespace ppsk_host {
+
+using sparse_coo = std::tuple<std::vector<long>, std::vector<long>, std::vector<float>>;
+
+// Row-major float matrix (the reference's NumpyMatrix, src/dist/matrix.hpp:25).
+struct DenseMatrix {
+  size_t rows = 0, cols = 0;
+  std::vector<float> data;
+};
+
+// Distances failed for some pair: the reference prints and exit(1)s the
+// process (src/dist/linear_regression.cpp:51-57); here it is an exception.
+struct FitError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// Regression tolerance of the reference: a function-local static set on the
+// first core/accessory fit of the process, 5 / (64 * sketchsize64 of that
+// call's first sample) (src/dist/linear_regression.cpp:22).
+double regression_tolerance(size_t sketchsize64);
+
+// query_db: self mode when ref and query hold the same names in the same
+// order (src/api.cpp:195); dense core/acc or per-k Jaccard.
+DenseMatrix query_db(std::vector<Sketch>& ref, std::vector<Sketch>& query,
+                     const std::vector<size_t>& kmer_lengths, RandomMC& random, bool jaccard,
+                     int device_id);
+
+// query_db_sparse (kNN, src/api.cpp:319-432).
+sparse_coo query_db_sparse(std::vector<Sketch>& ref, const std::vector<size_t>& kmer_lengths,
+                           RandomMC& random, bool jaccard, int kNN, size_t dist_col,
+                           int device_id);
+
+// Self threshold sparsification computed on the device (the
+// dist_cutoff > 0 route of querySelfSparse, src/sketchlib_bindings.cpp:159-169).
+sparse_coo query_db_threshold(std::vector<Sketch>& ref, const std::vector<size_t>& kmer_lengths,
+                              RandomMC& random, float cutoff, size_t dist_col, int device_id);
+
+// One random-corrected Jaccard (src/sketchlib_bindings.cpp:204-216): the
+// match count from the device, the double-precision correction on the host.
+double jaccard_dist(const Sketch& s1, const Sketch& s2, size_t k, const RandomMC& random,
+                    int device_id);
+
+// Matrix utilities (src/dist/matrix_ops.cpp:38-150, matrix_idx.hpp:13-50).
+DenseMatrix long_to_square(const float* rr, size_t n_rr, const float* qr, size_t n_qr,
+                           const float* qq, size_t n_qq, unsigned num_threads);
+std::vector<float> square_to_long(const float* square, size_t n, unsigned num_threads);
+sparse_coo sparsify_dists_by_threshold(const float* dense, size_t rows, size_t cols,
+                                       float cutoff, unsigned num_threads);
+
+// Synthetic database writer: seeded sketches in the database layout.
+void write_synthetic_db(const std::string& db_name, int64_t n_samples,
+                        const std::vector<size_t>& kmer_lengths, size_t sketchsize64,
+                        uint64_t seed, double dcore_max, double dacc_max, double length_mean,
+                        const std::string& name_prefix);
+
+}  // namespace ppsk_host

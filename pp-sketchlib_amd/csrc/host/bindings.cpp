@@ -1,0 +1,285 @@
+// pp_sketchlib: Python module with the reference's API for the distance path
+// (src/sketchlib_bindings.cpp:238-312): same function names, keyword
+// arguments, defaults, return types and error messages.  Distances are
+// computed by the HIP library (libppsk.so); `use_gpu` and `device_id` are
+// accepted as in the reference, and a call that the reference rejects on its
+// GPU path is rejected here in the same way.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstring>
+#include <iostream>
+
+#include "api.hpp"
+
+namespace py = pybind11;
+using namespace ppsk_host;
+
+#ifndef PPSK_VERSION_INFO
+#define PPSK_VERSION_INFO "2.1.5"
+#endif
+// Sketch-format identifier of pp-sketchlib 2.1.5 (sha1 of its sketcher
+// sources, src/Makefile:103), so databases from that version compare equal.
+#define PPSK_SKETCH_VERSION "445afa87649c32acbf9749c77231b5b0fcb08179"
+
+namespace {
+
+using farray = py::array_t<float, py::array::c_style>;
+
+farray to_numpy(DenseMatrix&& m) {
+  farray a({(py::ssize_t)m.rows, (py::ssize_t)m.cols});
+  if (!m.data.empty()) std::memcpy(a.mutable_data(), m.data.data(), m.data.size() * sizeof(float));
+  return a;
+}
+
+void require_ascending(const std::vector<size_t>& k) {
+  for (size_t i = 1; i < k.size(); ++i) {
+    if (k[i] <= k[i - 1]) throw std::runtime_error("Mismatching k-mer lengths in sketches");
+  }
+}
+
+// get_random (src/api.cpp:533-537, src/database/database.cpp:195-226)
+RandomMC get_random(const std::string& db_name, bool use_rc_default) {
+  RandomTables t = read_random(db_name + ".h5");
+  if (t.present) return RandomMC(t);
+  std::cerr << "Could not find random match chances in database, "
+               "calculating assuming equal base frequencies"
+            << std::endl;
+  return RandomMC(use_rc_default);
+}
+
+std::vector<Sketch> load_or_throw(const std::string& db, const std::vector<std::string>& names,
+                                  const std::vector<size_t>& klist) {
+  std::vector<Sketch> s = load_sketches(db, names, klist, false);
+  if (s.empty()) throw std::runtime_error("Query with empty ref or query list!");
+  return s;
+}
+
+py::tuple coo_to_py(sparse_coo&& c) {
+  return py::make_tuple(py::cast(std::get<0>(c)), py::cast(std::get<1>(c)),
+                        py::cast(std::get<2>(c)));
+}
+
+farray queryDatabase(const std::string& ref_db_name, const std::string& query_db_name,
+                     const std::vector<std::string>& ref_names,
+                     const std::vector<std::string>& query_names,
+                     std::vector<size_t> kmer_lengths, bool random_correct, bool jaccard,
+                     size_t num_threads, bool use_gpu, int device_id) {
+  (void)num_threads;
+  if (use_gpu && (jaccard || kmer_lengths.size() < 2)) {
+    throw std::runtime_error("Extracting Jaccard distances not supported on GPU");
+  }
+  require_ascending(kmer_lengths);
+  std::vector<Sketch> ref = load_or_throw(ref_db_name, ref_names, kmer_lengths);
+  std::vector<Sketch> query;
+  if (ref_db_name == query_db_name && ref_names == query_names) {
+    query = ref;
+  } else {
+    query = load_or_throw(query_db_name, query_names, kmer_lengths);
+  }
+  RandomMC random = random_correct ? get_random(ref_db_name, ref[0].use_rc) : RandomMC();
+  DenseMatrix d;
+  {
+    py::gil_scoped_release nogil;
+    d = query_db(ref, query, kmer_lengths, random, jaccard, device_id);
+  }
+  return to_numpy(std::move(d));
+}
+
+py::tuple querySelfSparse(const std::string& ref_db_name, const std::vector<std::string>& ref_names,
+                          std::vector<size_t> kmer_lengths, bool random_correct, bool jaccard,
+                          unsigned long kNN, float dist_cutoff, size_t dist_col,
+                          size_t num_threads, bool use_gpu, int device_id) {
+  (void)num_threads;
+  if (use_gpu && (jaccard || kmer_lengths.size() < 2 || dist_col > 1)) {
+    throw std::runtime_error("Extracting Jaccard distances not supported on GPU");
+  }
+  require_ascending(kmer_lengths);
+  std::vector<Sketch> ref = load_or_throw(ref_db_name, ref_names, kmer_lengths);
+  RandomMC random = random_correct ? get_random(ref_db_name, ref[0].use_rc) : RandomMC();
+  sparse_coo out;
+  {
+    py::gil_scoped_release nogil;
+    if (dist_cutoff > 0) {
+      out = query_db_threshold(ref, kmer_lengths, random, dist_cutoff, dist_col, device_id);
+    } else {
+      out = query_db_sparse(ref, kmer_lengths, random, jaccard, (int)kNN, dist_col, device_id);
+    }
+  }
+  return coo_to_py(std::move(out));
+}
+
+double jaccardDist(const std::string& db_name, const std::string& sample1,
+                   const std::string& sample2, size_t kmer_size, bool random_correct) {
+  std::vector<Sketch> s = load_sketches(db_name, {sample1, sample2}, {kmer_size}, false);
+  if (s.size() != 2) throw std::out_of_range("vector::_M_range_check");
+  RandomMC random = random_correct ? get_random(db_name, s[0].use_rc) : RandomMC();
+  random.check_present(s, true);
+  return jaccard_dist(s[0], s[1], kmer_size, random, 0);
+}
+
+std::vector<float> flat(const farray& a) {
+  return std::vector<float>(a.data(), a.data() + a.size());
+}
+
+farray longToSquare(const farray& distVec, unsigned num_threads) {
+  return to_numpy(long_to_square(distVec.data(), distVec.size(), nullptr, 0, nullptr, 0, num_threads));
+}
+
+farray longToSquareMulti(const farray& distVec, const farray& qr, const farray& qq,
+                         unsigned num_threads) {
+  return to_numpy(long_to_square(distVec.data(), distVec.size(), qr.data(), qr.size(), qq.data(),
+                                 qq.size(), num_threads));
+}
+
+farray squareToLong(const farray& distMat, unsigned num_threads) {
+  if (distMat.ndim() != 2 || distMat.shape(0) != distMat.shape(1)) {
+    throw std::runtime_error("square_to_long input must be a square matrix");
+  }
+  std::vector<float> v = square_to_long(distMat.data(), distMat.shape(0), num_threads);
+  farray a(v.size());
+  if (!v.empty()) std::memcpy(a.mutable_data(), v.data(), v.size() * sizeof(float));
+  return a;
+}
+
+py::tuple sparsifyDistsByThreshold(const farray& denseDists, float distCutoff, size_t num_threads) {
+  if (denseDists.ndim() != 2) throw std::runtime_error("distMat must be a 2D matrix");
+  return coo_to_py(sparsify_dists_by_threshold(denseDists.data(), denseDists.shape(0),
+                                               denseDists.shape(1), distCutoff,
+                                               (unsigned)num_threads));
+}
+
+py::tuple sparsifyDists(const farray&, float, int, size_t) {
+  throw std::runtime_error(
+      "pp_sketchlib.sparsifyDists was deprecated in v2.0.1\n"
+      "Downgrade to v2.0.0, or update the calling package");
+}
+
+void constructDatabase(const std::string&, const std::vector<std::string>&,
+                       const std::vector<std::vector<std::string>>&, std::vector<size_t>, size_t,
+                       bool, bool, bool, size_t, bool, size_t, bool, int) {
+  throw std::runtime_error(
+      "constructDatabase: sequence sketching is not part of this library, which provides the "
+      "distance path; sketch with pp-sketchlib or use writeSyntheticDatabase");
+}
+
+void addRandom(const std::string&, const std::vector<std::string>&, std::vector<size_t>, bool,
+               size_t) {
+  throw std::runtime_error(
+      "addRandom: building random-match tables is not part of this library, which provides "
+      "the distance path; databases without a /random group use the equal-base correction");
+}
+
+void writeSyntheticDatabase(const std::string& db_name, int64_t n_samples,
+                            std::vector<size_t> klist, size_t sketch_size, uint64_t seed,
+                            double dcore_max, double dacc_max, double length_mean,
+                            const std::string& name_prefix) {
+  require_ascending(klist);
+  // sketch_size in bins, rounded to 64-bin words as the reference CLI does
+  const size_t s64 = (size_t)std::llround(sketch_size / 64.0);
+  write_synthetic_db(db_name, n_samples, klist, s64, seed, dcore_max, dacc_max, length_mean,
+                     name_prefix);
+}
+
+void writeRandomMatches(const std::string& db_name, const std::vector<std::string>& table_keys,
+                        const std::vector<uint16_t>& table_values,
+                        const std::vector<uint64_t>& matches_keys,
+                        py::array_t<float, py::array::c_style | py::array::forcecast> matches,
+                        py::array_t<float, py::array::c_style | py::array::forcecast> centroids,
+                        unsigned k_min, unsigned k_max, bool use_rc) {
+  RandomTables t;
+  t.present = true;
+  t.use_rc = use_rc;
+  t.k_min = k_min;
+  t.k_max = k_max;
+  t.table_keys = table_keys;
+  t.table_values = table_values;
+  t.matches_keys = matches_keys;
+  if (matches.ndim() != 3 || (size_t)matches.shape(0) != matches_keys.size() ||
+      matches.shape(1) != matches.shape(2)) {
+    throw std::runtime_error("matches must be [n_keys][C][C]");
+  }
+  t.n_clusters = matches.shape(1);
+  if (centroids.ndim() != 2 || (size_t)centroids.shape(0) != t.n_clusters || centroids.shape(1) != 4) {
+    throw std::runtime_error("centroids must be [C][4]");
+  }
+  t.matches.assign(matches.data(), matches.data() + matches.size());
+  t.centroids.assign(centroids.data(), centroids.data() + centroids.size());
+  if (table_keys.size() != table_values.size()) throw std::runtime_error("table keys / values differ in length");
+  write_random(db_name + ".h5", t);
+}
+
+std::vector<std::string> getSampleNames(const std::string& db_name) {
+  return list_sketch_names(db_name + ".h5");
+}
+
+}  // namespace
+
+PYBIND11_MODULE(pp_sketchlib, m) {
+  m.doc() = "Sketch distances for PopPUNK (MI355X)";
+
+  m.def("constructDatabase", &constructDatabase, "Create and save sketches", py::arg("db_name"),
+        py::arg("samples"), py::arg("files"), py::arg("klist"), py::arg("sketch_size"),
+        py::arg("codon_phased") = false, py::arg("calc_random") = true, py::arg("use_rc") = true,
+        py::arg("min_count") = 0, py::arg("exact") = false, py::arg("num_threads") = 1,
+        py::arg("use_gpu") = false, py::arg("device_id") = 0);
+
+  m.def("queryDatabase", &queryDatabase, "Find distances between sketches; return all distances",
+        py::arg("ref_db_name"), py::arg("query_db_name"), py::arg("rList"), py::arg("qList"),
+        py::arg("klist"), py::arg("random_correct") = true, py::arg("jaccard") = false,
+        py::arg("num_threads") = 1, py::arg("use_gpu") = false, py::arg("device_id") = 0);
+
+  m.def("querySelfSparse", &querySelfSparse,
+        "Find self-self distances between sketches; return a sparse matrix",
+        py::arg("ref_db_name"), py::arg("rList"), py::arg("klist"),
+        py::arg("random_correct") = true, py::arg("jaccard") = false, py::arg("kNN") = 0,
+        py::arg("dist_cutoff") = 0.0f, py::arg("dist_col") = 0, py::arg("num_threads") = 1,
+        py::arg("use_gpu") = false, py::arg("device_id") = 0);
+
+  m.def("addRandom", &addRandom, "Add random match chances into older databases",
+        py::arg("db_name"), py::arg("samples"), py::arg("klist"), py::arg("use_rc") = true,
+        py::arg("num_threads") = 1);
+
+  m.def("jaccardDist", &jaccardDist, "Calculate a raw Jaccard distance", py::arg("db_name"),
+        py::arg("ref_name"), py::arg("query_name"), py::arg("kmer_length"),
+        py::arg("random_correct") = false);
+
+  m.def("squareToLong", &squareToLong, "Convert dense square matrices to long form",
+        py::arg("distMat").noconvert(), py::arg("num_threads") = 1);
+
+  m.def("longToSquare", &longToSquare, "Convert dense long form distance matrices to square form",
+        py::arg("distVec").noconvert(), py::arg("num_threads") = 1);
+
+  m.def("longToSquareMulti", &longToSquareMulti,
+        "Convert dense long form distance matrices to square form (in three blocks)",
+        py::arg("distVec").noconvert(), py::arg("query_ref_distVec").noconvert(),
+        py::arg("query_query_distVec").noconvert(), py::arg("num_threads") = 1);
+
+  m.def("sparsifyDistsByThreshold", &sparsifyDistsByThreshold,
+        "Transform all distances into a sparse matrix", py::arg("distMat").noconvert(),
+        py::arg("distCutoff") = 0, py::arg("num_threads") = 1);
+
+  m.def("sparsifyDists", &sparsifyDists, "Transform all distances into a sparse matrix",
+        py::arg("distMat").noconvert(), py::arg("distCutoff") = 0, py::arg("kNN") = 0,
+        py::arg("num_threads") = 1);
+
+  // additions of this library
+  m.def("writeSyntheticDatabase", &writeSyntheticDatabase,
+        "Write a database of seeded synthetic sketches (tree-evolved bins)", py::arg("db_name"),
+        py::arg("n_samples"), py::arg("klist"), py::arg("sketch_size") = 10000,
+        py::arg("seed") = 1, py::arg("dcore_max") = 0.05, py::arg("dacc_max") = 0.30,
+        py::arg("length_mean") = 2000000.0, py::arg("name_prefix") = "sample");
+  m.def("getSampleNames", &getSampleNames, "Sample names of a database, in HDF5 name order",
+        py::arg("db_name"));
+  m.def("writeRandomMatches", &writeRandomMatches,
+        "Store Monte Carlo random-match tables in a database's /random group",
+        py::arg("db_name"), py::arg("table_keys"), py::arg("table_values"),
+        py::arg("matches_keys"), py::arg("matches"), py::arg("centroids"), py::arg("k_min"),
+        py::arg("k_max"), py::arg("use_rc") = true);
+
+  m.attr("version") = PPSK_VERSION_INFO;
+  m.attr("sketchVersion") = PPSK_SKETCH_VERSION;
+
+  py::register_exception<HDF5Error>(m, "HDF5Exception");
+}

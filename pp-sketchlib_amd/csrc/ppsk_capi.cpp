@@ -38,11 +38,11 @@ int fail(int code, const std::string& msg) {
     }                                                                              \
   } while (0)
 
-// Per-device state: one stream, cached workspaces grown on demand.
+// Per-device state: a launch stream, a copy stream, cached workspaces grown on
+// demand.
 struct DeviceCtx {
   int device = -1;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void* counts = nullptr;
   size_t counts_bytes = 0;
   uint32_t* tile_done = nullptr;
@@ -52,6 +52,8 @@ struct DeviceCtx {
   unsigned long long* fail = nullptr;  // [0] count, [1] first row
   void* out = nullptr;
   size_t out_bytes = 0;
+  hipStream_t copy_stream = nullptr;  // device -> host copies of finished rows
+  std::vector<hipEvent_t> events;     // per-chunk timing / copy dependencies
   std::mutex mu;
 };
 
@@ -71,8 +73,7 @@ int get_ctx(int device, DeviceCtx** out) {
     auto* c = new DeviceCtx();
     c->device = device;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreate(&c->ev0));
-    HIP_TRY(hipEventCreate(&c->ev1));
+    HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&c->fail, 2 * sizeof(unsigned long long)));
     g_ctx[device] = c;
   }
@@ -136,19 +137,20 @@ int check_compatible(const ppsk_set* a, const ppsk_set* b) {
   return PPSK_OK;
 }
 
-// Tile lists.  Self: upper-triangle tiles (at least one pair with i < j).
-// Full square / query: every tile.  Row-major order (consecutive tiles share
-// the 128-row block).
-std::vector<ppsk::Tile> make_tiles(int64_t n_rows, int64_t n_cols, int tc, bool upper,
+// Tile lists for a tile shape of tr rows x tc columns.  Self: upper-triangle
+// tiles (at least one pair with i < j).  Full square / query: every tile.
+// Row-major order (consecutive tiles share their row block).  row_begin must
+// be a multiple of tr.
+std::vector<ppsk::Tile> make_tiles(int64_t n_rows, int64_t n_cols, int tr, int tc, bool upper,
                                    int64_t row_begin, int64_t row_end) {
   std::vector<ppsk::Tile> t;
-  const int64_t nbr = (n_rows + ppsk::BLK - 1) / ppsk::BLK;
+  const int64_t nbr = (n_rows + tr - 1) / tr;
   const int64_t nct = (n_cols + tc - 1) / tc;
-  const int64_t b0 = row_begin / ppsk::BLK;
-  const int64_t b1 = std::min(nbr, (row_end + ppsk::BLK - 1) / ppsk::BLK);
+  const int64_t b0 = row_begin / tr;
+  const int64_t b1 = std::min(nbr, (row_end + tr - 1) / tr);
   for (int64_t bi = b0; bi < b1; ++bi) {
     for (int64_t ct = 0; ct < nct; ++ct) {
-      if (upper && (ct + 1) * tc - 1 <= bi * ppsk::BLK) continue;
+      if (upper && (ct + 1) * tc - 1 <= bi * tr) continue;
       t.push_back({(int32_t)bi, (int32_t)ct});
     }
   }
@@ -182,19 +184,71 @@ int check_random(const ppsk_set* rows, const ppsk_set* cols, const RandomIn& rnd
   return PPSK_OK;
 }
 
-// Runs the distance tiles for `tiles` (chunked by count-scratch size) with
-// params p (tiles/counts/tile_done/item_base filled in here); accumulates
-// kernel time.
-int run_tiles(DeviceCtx* c, ppsk::DistParams p, const std::vector<ppsk::Tile>& tiles,
-              int rr, double* ms_acc, int64_t* launches, int64_t* items) {
+// Timing of the tile-kernel launches of one call: per-chunk events on the
+// launch stream, summed once the call has finished.
+struct Timing {
+  double ms = 0;
+  int64_t launches = 0;
+  int64_t items = 0;
+};
+
+hipEvent_t pool_event(DeviceCtx* c, size_t i) {
+  while (c->events.size() <= i) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->events.push_back(e);
+  }
+  return c->events[i];
+}
+
+// First output row of sample row r: self mode -> condensed index of (r, r+1),
+// query mode -> r * n_cols.
+int64_t out_row_start(bool self, int64_t r, int64_t n_rows, int64_t n_cols) {
+  if (!self) return std::min(r, n_rows) * n_cols;
+  const int64_t n = n_rows;
+  if (r >= n - 1) return n * (n - 1) / 2;
+  return n * r - ((r * (r + 1)) >> 1);
+}
+
+// Runs the tile kernel over `tiles` (row-major by row tile) writing dense
+// output into the device buffer p.out (row offset 0).  Chunks are cut at row
+// tile boundaries so that each fits the per-tile count scratch; when host_out
+// is set, the finished output rows of each chunk are copied to host_out on
+// the copy stream while the following chunks compute.  Returns after both
+// streams have drained.
+int run_dense(DeviceCtx* c, ppsk::DistParams p, int variant, const std::vector<ppsk::Tile>& tiles,
+              bool self, size_t elem, void* host_out, Timing* tm) {
   if (tiles.empty()) return PPSK_OK;
-  const int tc = 64 * rr;
-  const size_t per_tile = (size_t)p.n_k * ppsk::TROWS * tc * sizeof(uint16_t);
-  const size_t budget = (size_t)1 << 30;  // count scratch per launch
-  size_t chunk = std::max<size_t>(1, budget / per_tile);
-  chunk = std::min(chunk, tiles.size());
-  if (int rc = grow(&c->counts, &c->counts_bytes, chunk * per_tile)) return rc;
-  if (int rc = grow_tile_done(c, chunk)) return rc;
+  const int tr = ppsk::variant_tile_rows(variant);
+  const size_t per_tile =
+      (size_t)p.n_k * tr * ppsk::variant_tile_cols(variant) * sizeof(uint16_t);
+  // count scratch per launch; PPSK_CHUNK_BYTES lowers it (more, smaller
+  // chunks: finer-grained streaming of host output)
+  size_t budget = (size_t)1 << 30;
+  if (const char* env = std::getenv("PPSK_CHUNK_BYTES")) {
+    const long long v = std::atoll(env);
+    if (v > 0) budget = (size_t)v;
+  }
+  const size_t max_tiles = std::max<size_t>(1, budget / per_tile);
+  // chunk boundaries at row-tile changes
+  std::vector<std::pair<size_t, size_t>> chunks;
+  size_t t0 = 0;
+  while (t0 < tiles.size()) {
+    size_t t1 = t0;
+    while (t1 < tiles.size()) {
+      size_t e = t1;
+      while (e < tiles.size() && tiles[e].bi == tiles[t1].bi) ++e;  // one row tile
+      if (t1 > t0 && e - t0 > max_tiles) break;
+      t1 = e;
+      if (t1 - t0 >= max_tiles) break;
+    }
+    chunks.emplace_back(t0, t1);
+    t0 = t1;
+  }
+  size_t most = 0;
+  for (auto& ch : chunks) most = std::max(most, ch.second - ch.first);
+  if (int rc = grow(&c->counts, &c->counts_bytes, most * per_tile)) return rc;
+  if (int rc = grow_tile_done(c, most)) return rc;
   void* tptr = c->tiles;
   size_t tbytes = c->tiles_n * sizeof(ppsk::Tile);
   if (int rc = grow(&tptr, &tbytes, tiles.size() * sizeof(ppsk::Tile))) return rc;
@@ -204,21 +258,42 @@ int run_tiles(DeviceCtx* c, ppsk::DistParams p, const std::vector<ppsk::Tile>& t
                          hipMemcpyHostToDevice, c->stream));
   p.counts = (uint16_t*)c->counts;
   p.tile_done = c->tile_done;
-  for (size_t t0 = 0; t0 < tiles.size(); t0 += chunk) {
-    const size_t nt = std::min(chunk, tiles.size() - t0);
-    p.tiles = c->tiles + t0;
-    p.n_tiles = (int64_t)nt;
+  p.out_row_offset = 0;
+  char* dev = (char*)p.out;
+  for (size_t ci = 0; ci < chunks.size(); ++ci) {
+    const size_t a = chunks[ci].first, b = chunks[ci].second;
+    p.tiles = c->tiles + a;
+    p.n_tiles = (int64_t)(b - a);
     p.item_base = 0;
-    const int64_t n_items = (int64_t)nt * p.n_k;
-    HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    HIP_TRY(ppsk::launch_dists(p, rr, n_items, c->stream));
-    HIP_TRY(hipEventRecord(c->ev1, c->stream));
-    HIP_TRY(hipEventSynchronize(c->ev1));
+    const int64_t n_items = (int64_t)(b - a) * p.n_k;
+    hipEvent_t e0 = pool_event(c, 2 * ci), e1 = pool_event(c, 2 * ci + 1);
+    if (!e0 || !e1) return fail(PPSK_ERR_HIP, "event creation failed");
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    HIP_TRY(ppsk::launch_dists(p, variant, n_items, c->stream));
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    tm->launches += 1;
+    tm->items += n_items;
+  }
+  // Copies are queued after every launch: a copy into pageable memory may
+  // block this thread until it completes, which must not hold back launches.
+  for (size_t ci = 0; host_out && ci < chunks.size(); ++ci) {
+    const size_t a = chunks[ci].first, b = chunks[ci].second;
+    const int64_t r0 = (int64_t)tiles[a].bi * tr;
+    const int64_t r1 = (int64_t)(tiles[b - 1].bi + 1) * tr;
+    const int64_t o0 = out_row_start(self, r0, p.n_rows, p.n_cols);
+    const int64_t o1 = out_row_start(self, r1, p.n_rows, p.n_cols);
+    if (o1 > o0) {
+      HIP_TRY(hipStreamWaitEvent(c->copy_stream, c->events[2 * ci + 1], 0));
+      HIP_TRY(hipMemcpyAsync((char*)host_out + o0 * elem, dev + o0 * elem, (o1 - o0) * elem,
+                             hipMemcpyDeviceToHost, c->copy_stream));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipStreamSynchronize(c->copy_stream));
+  for (size_t ci = 0; ci < chunks.size(); ++ci) {
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    *ms_acc += ms;
-    *launches += 1;
-    *items += n_items;
+    HIP_TRY(hipEventElapsedTime(&ms, c->events[2 * ci], c->events[2 * ci + 1]));
+    tm->ms += ms;
   }
   return PPSK_OK;
 }
@@ -268,13 +343,132 @@ int read_fail(DeviceCtx* c, unsigned long long* cnt, unsigned long long* first) 
   return PPSK_OK;
 }
 
-int default_rr() {
-  static int rr = -1;
-  if (rr < 0) {
-    const char* e = std::getenv("PPSK_TILE_RR");
-    rr = (e && std::atoi(e) == 1) ? 1 : 2;
+// Tile kernel configuration: PPSK_TILE_VARIANT overrides the default.
+int default_variant() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("PPSK_TILE_VARIANT");
+    v = e ? std::atoi(e) : ppsk::DEFAULT_VARIANT;
+    if (!ppsk::variant_valid(v)) v = ppsk::DEFAULT_VARIANT;
   }
-  return rr;
+  return v;
+}
+
+std::vector<ppsk::Tile> variant_tiles(int variant, int64_t n_rows, int64_t n_cols, bool upper,
+                                      int64_t row_begin, int64_t row_end) {
+  return make_tiles(n_rows, n_cols, ppsk::variant_tile_rows(variant),
+                    ppsk::variant_tile_cols(variant), upper, row_begin, row_end);
+}
+
+bool table_symmetric(const RandomIn& rnd, int n_k) {
+  if (rnd.mode != PPSK_RANDOM_MC) return true;
+  const int C = rnd.n_clusters;
+  for (int k = 0; k < n_k; ++k) {
+    for (int a = 0; a < C; ++a) {
+      for (int b = a + 1; b < C; ++b) {
+        if (rnd.table[((size_t)k * C + a) * C + b] != rnd.table[((size_t)k * C + b) * C + a]) {
+          return false;
+        }
+      }
+    }
+  }
+  return true;
+}
+
+// One dense pass (self or query) into the device buffer dout, optionally
+// streamed to host_out; reports fit failures.
+int dense_pass(DeviceCtx* c, const ppsk_set* rows, const ppsk_set* cols, bool self,
+               const RandomIn& rnd, double tol, int32_t out_kind, void* dout, void* host_out,
+               size_t elem, Timing* tm, unsigned long long* nf, unsigned long long* ff) {
+  float* dtab = nullptr;
+  if (int rc = upload_table(c, rows, rnd, &dtab)) return rc;
+  ppsk::DistParams p;
+  fill_common(p, rows, cols, rnd, tol);
+  p.self_mode = self ? 1 : 0;
+  p.full_square = 0;
+  p.out_kind = out_kind;
+  p.out = dout;
+  p.mc_table = dtab;
+  p.mc_ref_is_col = self ? 0 : 1;  // query mode: random_match(r1 = ref, q)
+  p.fail_count = c->fail;
+  p.fail_first = c->fail + 1;
+  int rc = reset_fail(c);
+  if (rc == PPSK_OK) {
+    const int variant = default_variant();
+    auto tiles = variant_tiles(variant, rows->n, cols->n, self, 0, rows->n);
+    rc = run_dense(c, p, variant, tiles, self, elem, host_out, tm);
+  }
+  if (dtab) hipFree(dtab);
+  if (rc) return rc;
+  return read_fail(c, nf, ff);
+}
+
+// kNN selection over all rows of a symmetric self distance held as a
+// condensed triangle on the device; outputs copied to host.
+int knn_from_triangle(DeviceCtx* c, const float* tri, int64_t n, int col, int kNN, int64_t* i_vec,
+                      int64_t* j_vec, float* dists) {
+  int64_t *d_i = nullptr, *d_j = nullptr;
+  float* d_d = nullptr;
+  const size_t m = (size_t)n * kNN;
+  hipError_t e = hipMalloc(&d_i, m * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc(&d_j, m * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc(&d_d, m * sizeof(float));
+  if (e == hipSuccess) e = ppsk::launch_knn_select_triangle(tri, n, col, kNN, d_i, d_j, d_d, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(i_vec, d_i, m * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(j_vec, d_j, m * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dists, d_d, m * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (d_i) hipFree(d_i);
+  if (d_j) hipFree(d_j);
+  if (d_d) hipFree(d_d);
+  if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? PPSK_ERR_MEMORY : PPSK_ERR_HIP,
+                                   std::string("kNN selection: ") + hipGetErrorString(e));
+  return PPSK_OK;
+}
+
+// Threshold compaction of a device triangle into library-allocated host arrays.
+int threshold_from_triangle(DeviceCtx* c, const float2* tri, int64_t n, int col, float cutoff,
+                            int64_t** i_vec, int64_t** j_vec, float** dists, int64_t* n_out) {
+  int64_t total = 0;
+  int64_t *d_i = nullptr, *d_j = nullptr;
+  float* d_d = nullptr;
+  hipError_t e = ppsk::threshold_compact(tri, n, col, cutoff, &total, &d_i, &d_j, &d_d, c->stream);
+  if (e != hipSuccess) return fail(PPSK_ERR_HIP, std::string("threshold: ") + hipGetErrorString(e));
+  auto* hi = (int64_t*)std::malloc(std::max<int64_t>(total, 1) * sizeof(int64_t));
+  auto* hj = (int64_t*)std::malloc(std::max<int64_t>(total, 1) * sizeof(int64_t));
+  auto* hd = (float*)std::malloc(std::max<int64_t>(total, 1) * sizeof(float));
+  if (hi && hj && hd && total > 0) {
+    e = hipMemcpy(hi, d_i, total * sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hj, d_j, total * sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hd, d_d, total * sizeof(float), hipMemcpyDeviceToHost);
+  }
+  if (d_i) hipFree(d_i);
+  if (d_j) hipFree(d_j);
+  if (d_d) hipFree(d_d);
+  if (!hi || !hj || !hd || e != hipSuccess) {
+    std::free(hi);
+    std::free(hj);
+    std::free(hd);
+    if (e != hipSuccess) return fail(PPSK_ERR_HIP, std::string("threshold copy: ") + hipGetErrorString(e));
+    return fail(PPSK_ERR_MEMORY, "threshold host output");
+  }
+  *i_vec = hi;
+  *j_vec = hj;
+  *dists = hd;
+  *n_out = total;
+  return PPSK_OK;
+}
+
+int fit_status(unsigned long long nf, unsigned long long ff) {
+  if (!nf) return PPSK_OK;
+  return fail(PPSK_ERR_FIT, "Fitting k-mer gradient failed for " + std::to_string(nf) +
+                                " pair(s); first at row " + std::to_string(ff));
+}
+
+void set_timing(const Timing& tm) {
+  g_last_ms = tm.ms;
+  g_last_launches = tm.launches;
+  g_last_items = tm.items;
 }
 
 }  // namespace
@@ -326,7 +520,8 @@ int ppsk_set_create(int device_id, int64_t n_samples, int32_t n_k, const int32_t
   s->s64 = sketchsize64;
   s->bbits = bbits;
   s->nb = (int32_t)((n_samples + ppsk::BLK - 1) / ppsk::BLK);
-  const size_t bytes = (size_t)n_k * s->nb * sketchsize64 * ppsk::HL * ppsk::BLK * sizeof(uint32_t);
+  const size_t bytes = ((size_t)n_k * s->nb * sketchsize64 + ppsk::STRIPE_PAD_WORDS) *
+                       ppsk::HL * ppsk::BLK * sizeof(uint32_t);
   hipError_t e = hipMalloc(&s->stripes, bytes);
   if (e != hipSuccess) {
     delete s;
@@ -425,9 +620,7 @@ int64_t ppsk_set_size(const ppsk_set* s) { return s ? s->n : 0; }
 int ppsk_dense(ppsk_set* ref, ppsk_set* query, int32_t random_mode, int32_t n_clusters,
                const float* mc_table, double tolerance, int32_t out_kind, void* out,
                int32_t out_on_device, uint64_t* n_fail, int64_t* first_fail_row) {
-  g_last_ms = 0;
-  g_last_launches = 0;
-  g_last_items = 0;
+  set_timing(Timing());
   if (int rc = check_set(ref, "ref")) return rc;
   const bool self = (query == nullptr || query == ref);
   if (!self) {
@@ -452,63 +645,38 @@ int ppsk_dense(ppsk_set* ref, ppsk_set* query, int32_t random_mode, int32_t n_cl
   const size_t elem = out_kind == PPSK_OUT_CORE_ACC ? 2 * sizeof(float)
                       : out_kind == PPSK_OUT_JACCARD ? ref->n_k * sizeof(float)
                                                      : ref->n_k * sizeof(uint16_t);
-  const size_t out_bytes = (size_t)n_rows_out * elem;
-  if (n_rows_out == 0) {
-    if (n_fail) *n_fail = 0;
-    if (first_fail_row) *first_fail_row = -1;
-    return PPSK_OK;
-  }
+  if (n_fail) *n_fail = 0;
+  if (first_fail_row) *first_fail_row = -1;
+  if (n_rows_out == 0) return PPSK_OK;
   void* dout = out;
   if (!out_on_device) {
-    if (int rc = grow(&c->out, &c->out_bytes, out_bytes)) return rc;
+    if (int rc = grow(&c->out, &c->out_bytes, (size_t)n_rows_out * elem)) return rc;
     dout = c->out;
   }
-  float* dtab = nullptr;
-  if (int rc = upload_table(c, ref, rnd, &dtab)) return rc;
-
-  ppsk::DistParams p;
-  fill_common(p, rows, cols, rnd, tolerance);
-  p.self_mode = self ? 1 : 0;
-  p.full_square = 0;
-  p.out_kind = out_kind;
-  p.out = dout;
-  p.mc_table = dtab;
-  p.mc_ref_is_col = self ? 0 : 1;  // query mode: random_match(r1 = ref, q)
-  p.fail_count = c->fail;
-  p.fail_first = c->fail + 1;
-  if (int rc = reset_fail(c)) return rc;
-
-  const int rr = default_rr();
-  auto tiles = make_tiles(rows->n, cols->n, 64 * rr, self, 0, rows->n);
-  int rc = run_tiles(c, p, tiles, rr, &g_last_ms, &g_last_launches, &g_last_items);
-  if (dtab) hipFree(dtab);
-  if (rc) return rc;
+  Timing tm;
   unsigned long long nf = 0, ff = 0;
-  if (int e = read_fail(c, &nf, &ff)) return e;
-  if (!out_on_device) {
-    HIP_TRY(hipMemcpy(out, dout, out_bytes, hipMemcpyDeviceToHost));
-  }
+  int rc = dense_pass(c, rows, cols, self, rnd, tolerance, out_kind, dout,
+                      out_on_device ? nullptr : out, elem, &tm, &nf, &ff);
+  set_timing(tm);
+  if (rc) return rc;
   if (n_fail) *n_fail = nf;
   if (first_fail_row) *first_fail_row = nf ? (int64_t)ff : -1;
-  if (nf) {
-    return fail(PPSK_ERR_FIT, "Fitting k-mer gradient failed for " + std::to_string(nf) +
-                                  " pair(s); first at row " + std::to_string(ff));
-  }
-  return PPSK_OK;
+  return fit_status(nf, ff);
 }
 
 int ppsk_knn(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float* mc_table,
-             double tolerance, int32_t kNN, int32_t dist_col, int64_t* i_vec, int64_t* j_vec,
-             float* dists, uint64_t* n_fail) {
-  g_last_ms = 0;
-  g_last_launches = 0;
-  g_last_items = 0;
+             double tolerance, int32_t jaccard, int32_t kNN, int32_t dist_col, int64_t* i_vec,
+             int64_t* j_vec, float* dists, uint64_t* n_fail) {
+  set_timing(Timing());
   if (int rc = check_set(ref, "ref")) return rc;
-  if (kNN < 1 || kNN > ref->n) {
-    return fail(PPSK_ERR_ARG, "kNN must be in [1, number of samples]");
+  if (kNN < 1 || kNN > ref->n || kNN > ppsk::KNN_MAX) {
+    return fail(PPSK_ERR_ARG, "kNN must be in [1, min(number of samples, " +
+                                  std::to_string(ppsk::KNN_MAX) + ")]");
   }
-  if (dist_col != 0 && dist_col != 1) return fail(PPSK_ERR_ARG, "dist_col out of range");
-  if (ref->n_k < 2) {
+  if ((jaccard && dist_col >= ref->n_k) || (dist_col != 0 && dist_col != 1)) {
+    return fail(PPSK_ERR_ARG, "dist_col out of range");
+  }
+  if (!jaccard && ref->n_k < 2) {
     return fail(PPSK_ERR_ARG, "core/accessory distances need at least two k-mer lengths");
   }
   if (!i_vec || !j_vec || !dists) return fail(PPSK_ERR_ARG, "output is NULL");
@@ -518,148 +686,184 @@ int ppsk_knn(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float
   if (int rc = get_ctx(ref->device, &c)) return rc;
   std::lock_guard<std::mutex> lk(c->mu);
   const int64_t n = ref->n;
-  const int rr = default_rr();
-  // bands of rows: square block [band][n] float2 (core, acc)
-  int64_t band = std::max<int64_t>(ppsk::BLK, ((int64_t)(2ll << 30) / (n * 8)) / ppsk::BLK * ppsk::BLK);
-  band = std::min<int64_t>(band, (n + ppsk::BLK - 1) / ppsk::BLK * ppsk::BLK);
-  if (int rc = grow(&c->out, &c->out_bytes, (size_t)band * n * 2 * sizeof(float))) return rc;
-  float* dtab = nullptr;
-  if (int rc = upload_table(c, ref, rnd, &dtab)) return rc;
-  int64_t* d_i = nullptr;
-  int64_t* d_j = nullptr;
-  float* d_d = nullptr;
+  const int32_t kind = jaccard ? ppsk::OUT_ONE_MINUS_J : PPSK_OUT_CORE_ACC;
+  Timing tm;
+  unsigned long long nf = 0, ff = 0;
   int rc = PPSK_OK;
-  auto cleanup = [&]() {
-    if (dtab) hipFree(dtab);
-    if (d_i) hipFree(d_i);
-    if (d_j) hipFree(d_j);
-    if (d_d) hipFree(d_d);
-  };
-  if (hipMalloc(&d_i, (size_t)band * kNN * sizeof(int64_t)) != hipSuccess ||
-      hipMalloc(&d_j, (size_t)band * kNN * sizeof(int64_t)) != hipSuccess ||
-      hipMalloc(&d_d, (size_t)band * kNN * sizeof(float)) != hipSuccess) {
-    cleanup();
-    return fail(PPSK_ERR_MEMORY, "kNN output buffers");
-  }
-  if ((rc = reset_fail(c))) {
-    cleanup();
-    return rc;
-  }
-  for (int64_t r0 = 0; r0 < n && rc == PPSK_OK; r0 += band) {
-    const int64_t r1 = std::min(n, r0 + band);
-    ppsk::DistParams p;
-    fill_common(p, ref, ref, rnd, tolerance);
-    p.self_mode = 1;
-    p.full_square = 1;
-    p.out_kind = PPSK_OUT_CORE_ACC;
-    p.out = c->out;
-    p.out_stride_rows = n;
-    p.square_row0 = r0;
-    p.mc_table = dtab;
-    p.mc_ref_is_col = 0;
-    p.fail_count = c->fail;
-    p.fail_first = c->fail + 1;
-    auto tiles = make_tiles(n, n, 64 * rr, false, r0, r1);
-    rc = run_tiles(c, p, tiles, rr, &g_last_ms, &g_last_launches, &g_last_items);
-    if (rc) break;
-    hipError_t e = ppsk::launch_knn_select((const float*)c->out, r0, r1 - r0, n, dist_col, kNN,
-                                           d_i, d_j, d_d, c->stream);
-    if (e == hipSuccess) {
+  const size_t tri_bytes = (size_t)n * (n - 1) / 2 * sizeof(float2);
+  if (table_symmetric(rnd, ref->n_k) && tri_bytes <= ((size_t)96 << 30)) {
+    // distances are symmetric: compute the upper triangle once and read both
+    // (i, j) and (j, i) from it
+    if ((rc = grow(&c->out, &c->out_bytes, std::max<size_t>(tri_bytes, 8)))) return rc;
+    if (n > 1) {
+      rc = dense_pass(c, ref, ref, true, rnd, tolerance, kind, c->out, nullptr, sizeof(float2),
+                      &tm, &nf, &ff);
+    }
+    if (rc == PPSK_OK) {
+      rc = knn_from_triangle(c, (const float*)c->out, n, dist_col, kNN, i_vec, j_vec, dists);
+    }
+  } else {
+    // asymmetric Monte Carlo table: full square rows in bands
+    const int variant = default_variant();
+    int64_t band = std::max<int64_t>(ppsk::BLK, ((int64_t)(2ll << 30) / (n * 8)) / ppsk::BLK * ppsk::BLK);
+    band = std::min<int64_t>(band, (n + ppsk::BLK - 1) / ppsk::BLK * ppsk::BLK);
+    if ((rc = grow(&c->out, &c->out_bytes, (size_t)band * n * 2 * sizeof(float)))) return rc;
+    float* dtab = nullptr;
+    if ((rc = upload_table(c, ref, rnd, &dtab))) return rc;
+    int64_t *d_i = nullptr, *d_j = nullptr;
+    float* d_d = nullptr;
+    hipError_t e = hipMalloc(&d_i, (size_t)band * kNN * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc(&d_j, (size_t)band * kNN * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc(&d_d, (size_t)band * kNN * sizeof(float));
+    if (e != hipSuccess) rc = fail(PPSK_ERR_MEMORY, "kNN output buffers");
+    if (rc == PPSK_OK) rc = reset_fail(c);
+    for (int64_t r0 = 0; r0 < n && rc == PPSK_OK; r0 += band) {
+      const int64_t r1 = std::min(n, r0 + band);
+      ppsk::DistParams p;
+      fill_common(p, ref, ref, rnd, tolerance);
+      p.self_mode = 1;
+      p.full_square = 1;
+      p.out_kind = kind;
+      p.out = c->out;
+      p.out_stride_rows = n;
+      p.square_row0 = r0;
+      p.mc_table = dtab;
+      p.mc_ref_is_col = 0;
+      p.fail_count = c->fail;
+      p.fail_first = c->fail + 1;
+      auto tiles = variant_tiles(variant, n, n, false, r0, r1);
+      rc = run_dense(c, p, variant, tiles, true, sizeof(float2), nullptr, &tm);
+      if (rc) break;
+      e = ppsk::launch_knn_select((const float*)c->out, r0, r1 - r0, n, dist_col, kNN, d_i, d_j,
+                                  d_d, c->stream);
       const size_t m = (size_t)(r1 - r0) * kNN;
-      e = hipMemcpyAsync(i_vec + r0 * kNN, d_i, m * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(i_vec + r0 * kNN, d_i, m * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
       if (e == hipSuccess)
         e = hipMemcpyAsync(j_vec + r0 * kNN, d_j, m * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
       if (e == hipSuccess)
         e = hipMemcpyAsync(dists + r0 * kNN, d_d, m * sizeof(float), hipMemcpyDeviceToHost, c->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) rc = fail(PPSK_ERR_HIP, std::string("kNN select: ") + hipGetErrorString(e));
     }
-    if (e != hipSuccess) rc = fail(PPSK_ERR_HIP, std::string("kNN select: ") + hipGetErrorString(e));
+    if (dtab) hipFree(dtab);
+    if (d_i) hipFree(d_i);
+    if (d_j) hipFree(d_j);
+    if (d_d) hipFree(d_d);
+    if (rc == PPSK_OK) rc = read_fail(c, &nf, &ff);
   }
-  cleanup();
+  set_timing(tm);
   if (rc) return rc;
-  unsigned long long nf = 0, ff = 0;
-  if (int e = read_fail(c, &nf, &ff)) return e;
-  if (n_fail) *n_fail = nf;
-  if (nf) {
-    return fail(PPSK_ERR_FIT, "Fitting k-mer gradient failed for " + std::to_string(nf) +
-                                  " pair(s)");
+  if (jaccard) {
+    // ranked on 1 - J, reported as J (src/api.cpp:423-429)
+    for (int64_t x = 0; x < n * (int64_t)kNN; ++x) dists[x] = 1.0f - dists[x];
   }
-  return PPSK_OK;
+  if (n_fail) *n_fail = nf;
+  return fit_status(nf, ff);
 }
 
 int ppsk_threshold(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float* mc_table,
                    double tolerance, int32_t dist_col, float cutoff, int64_t** i_vec,
                    int64_t** j_vec, float** dists, int64_t* n_out, uint64_t* n_fail) {
+  set_timing(Timing());
   if (int rc = check_set(ref, "ref")) return rc;
   if (!(cutoff >= 0)) return fail(PPSK_ERR_ARG, "Distance threshold must be at least 0");
   if (dist_col != 0 && dist_col != 1) return fail(PPSK_ERR_ARG, "dist_col out of range");
   if (!i_vec || !j_vec || !dists || !n_out) return fail(PPSK_ERR_ARG, "output is NULL");
+  if (ref->n_k < 2) {
+    return fail(PPSK_ERR_ARG, "core/accessory distances need at least two k-mer lengths");
+  }
   *i_vec = nullptr;
   *j_vec = nullptr;
   *dists = nullptr;
   *n_out = 0;
-  const int64_t n = ref->n;
-  const int64_t rows = n * (n - 1) / 2;
+  RandomIn rnd{random_mode, n_clusters, mc_table};
+  if (int rc = check_random(ref, ref, rnd)) return rc;
   DeviceCtx* c = nullptr;
   if (int rc = get_ctx(ref->device, &c)) return rc;
-  float2* dd = nullptr;
-  HIP_TRY(hipMalloc(&dd, std::max<int64_t>(rows, 1) * sizeof(float2)));
-  uint64_t nf = 0;
-  int64_t ff = -1;
-  int rc = ppsk_dense(ref, nullptr, random_mode, n_clusters, mc_table, tolerance,
-                      PPSK_OUT_CORE_ACC, dd, 1, &nf, &ff);
-  if (n_fail) *n_fail = nf;
-  if (rc) {
-    hipFree(dd);
-    return rc;
-  }
-  double ms = g_last_ms;
-  int64_t nl = g_last_launches, ni = g_last_items;
   std::lock_guard<std::mutex> lk(c->mu);
-  int64_t total = 0;
-  int64_t *d_i = nullptr, *d_j = nullptr;
-  float* d_d = nullptr;
-  hipError_t e = ppsk::threshold_compact(dd, n, dist_col, cutoff, &total, &d_i, &d_j, &d_d,
-                                         c->stream);
-  hipFree(dd);
-  if (e != hipSuccess) return fail(PPSK_ERR_HIP, std::string("threshold: ") + hipGetErrorString(e));
-  auto* hi = (int64_t*)std::malloc(std::max<int64_t>(total, 1) * sizeof(int64_t));
-  auto* hj = (int64_t*)std::malloc(std::max<int64_t>(total, 1) * sizeof(int64_t));
-  auto* hd = (float*)std::malloc(std::max<int64_t>(total, 1) * sizeof(float));
-  if (!hi || !hj || !hd) {
-    std::free(hi);
-    std::free(hj);
-    std::free(hd);
-    hipFree(d_i);
-    hipFree(d_j);
-    hipFree(d_d);
-    return fail(PPSK_ERR_MEMORY, "threshold host output");
+  const int64_t n = ref->n;
+  const size_t tri_bytes = std::max<size_t>((size_t)n * (n - 1) / 2 * sizeof(float2), 8);
+  if (int rc = grow(&c->out, &c->out_bytes, tri_bytes)) return rc;
+  Timing tm;
+  unsigned long long nf = 0, ff = 0;
+  int rc = PPSK_OK;
+  if (n > 1) {
+    rc = dense_pass(c, ref, ref, true, rnd, tolerance, PPSK_OUT_CORE_ACC, c->out, nullptr,
+                    sizeof(float2), &tm, &nf, &ff);
   }
-  if (total > 0) {
-    e = hipMemcpy(hi, d_i, total * sizeof(int64_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(hj, d_j, total * sizeof(int64_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(hd, d_d, total * sizeof(float), hipMemcpyDeviceToHost);
+  set_timing(tm);
+  if (n_fail) *n_fail = nf;
+  if (rc) return rc;
+  if (nf) return fit_status(nf, ff);
+  return threshold_from_triangle(c, (const float2*)c->out, n, dist_col, cutoff, i_vec, j_vec,
+                                 dists, n_out);
+}
+
+int ppsk_self_pass(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float* mc_table,
+                   double tolerance, float* dense_out, int32_t kNN, int32_t knn_col,
+                   int64_t* knn_i, int64_t* knn_j, float* knn_d, float cutoff, int32_t thr_col,
+                   int64_t** thr_i, int64_t** thr_j, float** thr_d, int64_t* thr_n,
+                   uint64_t* n_fail) {
+  set_timing(Timing());
+  if (int rc = check_set(ref, "ref")) return rc;
+  if (ref->n_k < 2) {
+    return fail(PPSK_ERR_ARG, "core/accessory distances need at least two k-mer lengths");
   }
-  hipFree(d_i);
-  hipFree(d_j);
-  hipFree(d_d);
-  if (e != hipSuccess) {
-    std::free(hi);
-    std::free(hj);
-    std::free(hd);
-    return fail(PPSK_ERR_HIP, std::string("threshold copy: ") + hipGetErrorString(e));
+  if (kNN < 0 || kNN > ref->n || kNN > ppsk::KNN_MAX) {
+    return fail(PPSK_ERR_ARG, "kNN must be in [0, min(number of samples, " +
+                                  std::to_string(ppsk::KNN_MAX) + ")]");
   }
-  *i_vec = hi;
-  *j_vec = hj;
-  *dists = hd;
-  *n_out = total;
-  g_last_ms = ms;
-  g_last_launches = nl;
-  g_last_items = ni;
-  return PPSK_OK;
+  if (kNN > 0 && ((knn_col != 0 && knn_col != 1) || !knn_i || !knn_j || !knn_d)) {
+    return fail(PPSK_ERR_ARG, "bad kNN output arguments");
+  }
+  if (cutoff > 0 && ((thr_col != 0 && thr_col != 1) || !thr_i || !thr_j || !thr_d || !thr_n)) {
+    return fail(PPSK_ERR_ARG, "bad threshold output arguments");
+  }
+  RandomIn rnd{random_mode, n_clusters, mc_table};
+  if (int rc = check_random(ref, ref, rnd)) return rc;
+  if (!table_symmetric(rnd, ref->n_k)) {
+    return fail(PPSK_ERR_ARG, "a single self pass needs symmetric random-match tables");
+  }
+  DeviceCtx* c = nullptr;
+  if (int rc = get_ctx(ref->device, &c)) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  const int64_t n = ref->n;
+  const size_t tri_bytes = std::max<size_t>((size_t)n * (n - 1) / 2 * sizeof(float2), 8);
+  if (int rc = grow(&c->out, &c->out_bytes, tri_bytes)) return rc;
+  Timing tm;
+  unsigned long long nf = 0, ff = 0;
+  int rc = PPSK_OK;
+  if (n > 1) {
+    rc = dense_pass(c, ref, ref, true, rnd, tolerance, PPSK_OUT_CORE_ACC, c->out, dense_out,
+                    sizeof(float2), &tm, &nf, &ff);
+  }
+  set_timing(tm);
+  if (n_fail) *n_fail = nf;
+  if (rc) return rc;
+  if (nf) return fit_status(nf, ff);
+  if (kNN > 0) {
+    if ((rc = knn_from_triangle(c, (const float*)c->out, n, knn_col, kNN, knn_i, knn_j, knn_d))) return rc;
+  }
+  if (cutoff > 0) {
+    rc = threshold_from_triangle(c, (const float2*)c->out, n, thr_col, cutoff, thr_i, thr_j,
+                                 thr_d, thr_n);
+  }
+  return rc;
 }
 
 void ppsk_free_host(void* ptr) { std::free(ptr); }
+
+int ppsk_host_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return fail(PPSK_ERR_ARG, "ptr is NULL");
+  HIP_TRY(hipHostMalloc(ptr, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+  return PPSK_OK;
+}
+
+int ppsk_host_free(void* ptr) {
+  if (ptr) HIP_TRY(hipHostFree(ptr));
+  return PPSK_OK;
+}
 
 int ppsk_device_alloc(int device_id, size_t bytes, void** ptr) {
   DeviceCtx* c = nullptr;

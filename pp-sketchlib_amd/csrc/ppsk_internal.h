@@ -21,15 +21,17 @@ namespace ppsk {
 constexpr int BLK = 128;       // samples per stripe block
 constexpr int BBITS = 14;      // bit layers per bin value (reference def_bbits)
 constexpr int HL = 2 * BBITS;  // 32-bit planes per 64-bin word
-constexpr int RQ = 16;         // tile rows per wave (broadcast from SGPRs)
 constexpr int WAVES = 8;       // waves per workgroup
-constexpr int TROWS = RQ * WAVES;  // 128 tile rows
 constexpr int MAXK = 32;       // most k-mer lengths per query
+// zero words appended to every stripe buffer: the tile kernel stages whole
+// groups of up to 8 words and may read one group past the last word
+constexpr int STRIPE_PAD_WORDS = 8;
 
-enum OutKind : int32_t { OUT_CORE_ACC = 0, OUT_JACCARD = 1, OUT_COUNTS = 2 };
+enum OutKind : int32_t { OUT_CORE_ACC = 0, OUT_JACCARD = 1, OUT_COUNTS = 2, OUT_ONE_MINUS_J = 3 };
 enum RandomMode : int32_t { RANDOM_NONE = 0, RANDOM_EQUAL_BASE = 1, RANDOM_MC = 2 };
 
-// Tile: rows [bi*128, +128) x cols [ct*TC, +TC), TC = 64 * RR.
+// Tile: rows [bi * rows, +rows) x cols [ct * cols, +cols) of the variant's
+// tile shape (variant_tile_rows / variant_tile_cols).
 struct Tile {
   int32_t bi;
   int32_t ct;
@@ -46,7 +48,7 @@ struct DistParams {
   const Tile* tiles;
   int64_t n_tiles;
   int64_t item_base;            // first (tile * n_k + kk) item of this launch
-  uint16_t* counts;             // scratch [n_tiles][n_k][TROWS][TC]
+  uint16_t* counts;             // scratch [n_tiles][n_k][tile rows][tile cols]
   uint32_t* tile_done;          // [n_tiles] completion counters, left at zero
   // epilogue
   int32_t out_kind;
@@ -55,7 +57,6 @@ struct DistParams {
   int64_t out_stride_rows;      // for full_square: row stride of the [rows][n] block
   int64_t square_row0;          // for full_square: first global row in out
   float kvals[MAXK];            // k as float (reference kmer2mat)
-  double inv_bins;              // unused placeholder (kept 0)
   double tolerance;             // regression validity threshold
   int32_t random_mode;
   int32_t n_clusters;
@@ -73,7 +74,18 @@ struct DistParams {
 hipError_t launch_stripe(const uint64_t* raw, int64_t count, int64_t first,
                          int32_t n_k, int32_t s64, int32_t nb, uint32_t* stripes,
                          hipStream_t stream);
-hipError_t launch_dists(const DistParams& p, int rr, int64_t n_items,
+
+// Tile kernel configurations ("variants"): rows per wave RQ (row operands in
+// SGPRs), columns per lane RR, row-operand pipeline PB (0 = left to the
+// compiler, 1 or 2 = explicit double-buffered scalar loads of PB planes).
+//   1: RQ16 RR1 PB0 (128 x 64)     3: RQ16 RR2 PB0 (128 x 128)
+//   2: RQ8  RR4 PB2 (64 x 256)     4: RQ16 RR2 PB1 (128 x 128)
+//   5: RQ8  RR2 PB2 (64 x 128)
+constexpr int DEFAULT_VARIANT = 2;
+bool variant_valid(int variant);
+int variant_tile_rows(int variant);
+int variant_tile_cols(int variant);
+hipError_t launch_dists(const DistParams& p, int variant, int64_t n_items,
                         hipStream_t stream);
 
 }  // namespace ppsk

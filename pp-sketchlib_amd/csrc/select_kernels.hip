@@ -70,22 +70,42 @@ __device__ __forceinline__ void heap_pop(HeapEnt* h, int len) {
   heap_sift_up(h, hole, 0, v);
 }
 
-__global__ __launch_bounds__(64) void knn_select_kernel(const float* __restrict__ band,
-                                                        int64_t r0, int64_t n, int col,
+// Row accessors: distance of (row gi, column j != gi) in column `col` of
+// float2 (core, acc) data.
+struct BandRows {  // square rows [r0, r0 + rows) x n, row stride n
+  const float* band;
+  int64_t r0, n;
+  int col;
+  __device__ float operator()(int64_t gi, int64_t j) const {
+    return band[((gi - r0) * n + j) * 2 + col];
+  }
+};
+
+struct TriangleRows {  // condensed upper triangle of n samples (symmetric distances)
+  const float* tri;
+  int64_t n;
+  int col;
+  __device__ float operator()(int64_t gi, int64_t j) const {
+    const int64_t a = gi < j ? gi : j, b = gi < j ? j : gi;
+    return tri[(n * a - ((a * (a + 1)) >> 1) + b - 1 - a) * 2 + col];
+  }
+};
+
+template <typename Rows>
+__global__ __launch_bounds__(64) void knn_select_kernel(Rows rows, int64_t r0, int64_t n,
                                                         int kNN, int64_t* __restrict__ iv,
                                                         int64_t* __restrict__ jv,
                                                         float* __restrict__ dv) {
   extern __shared__ HeapEnt heap[];
   const int64_t row = blockIdx.x;
   const int64_t gi = r0 + row;
-  const float* rp = band + row * n * 2 + col;
   const int lane = threadIdx.x;
   int len = 0;        // wave-uniform heap size
   float top = 0.0f;   // heap top once len == kNN
   for (int64_t c0 = 0; c0 < n; c0 += 64) {
     const int64_t j = c0 + lane;
     float d = INFINITY;
-    if (j < n && j != gi) d = rp[2 * j];
+    if (j < n && j != gi) d = rows(gi, j);
     const bool cand = (j < n) && (len < kNN || d < top);
     unsigned long long mask = __ballot(cand);
     while (mask) {
@@ -124,8 +144,18 @@ hipError_t launch_knn_select(const float* band, int64_t r0, int64_t n_rows, int6
   if (kNN < 1 || kNN > KNN_MAX || kNN > n) return hipErrorInvalidValue;
   if (n_rows <= 0) return hipSuccess;
   // push writes slot kNN before the pop trims the heap back to kNN entries
-  knn_select_kernel<<<(unsigned)n_rows, 64, (kNN + 1) * sizeof(HeapEnt), stream>>>(
-      band, r0, n, dist_col, kNN, i_vec, j_vec, dists);
+  knn_select_kernel<BandRows><<<(unsigned)n_rows, 64, (kNN + 1) * sizeof(HeapEnt), stream>>>(
+      BandRows{band, r0, n, dist_col}, r0, n, kNN, i_vec, j_vec, dists);
+  return hipGetLastError();
+}
+
+hipError_t launch_knn_select_triangle(const float* tri, int64_t n, int dist_col, int kNN,
+                                      int64_t* i_vec, int64_t* j_vec, float* dists,
+                                      hipStream_t stream) {
+  if (kNN < 1 || kNN > KNN_MAX || kNN > n) return hipErrorInvalidValue;
+  if (n <= 0) return hipSuccess;
+  knn_select_kernel<TriangleRows><<<(unsigned)n, 64, (kNN + 1) * sizeof(HeapEnt), stream>>>(
+      TriangleRows{tri, n, dist_col}, 0, n, kNN, i_vec, j_vec, dists);
   return hipGetLastError();
 }
 

@@ -27,8 +27,9 @@ _u64 = _c.c_uint64
 EXPORTS = [
     "ppsk_last_error", "ppsk_version", "ppsk_device_init", "ppsk_set_create",
     "ppsk_set_upload", "ppsk_set_meta", "ppsk_set_free", "ppsk_set_size", "ppsk_dense",
-    "ppsk_knn", "ppsk_threshold", "ppsk_free_host", "ppsk_device_alloc",
-    "ppsk_device_free", "ppsk_copy_to_host", "ppsk_last_timing", "ppsk_synth_sketches",
+    "ppsk_knn", "ppsk_threshold", "ppsk_self_pass", "ppsk_free_host", "ppsk_host_alloc",
+    "ppsk_host_free", "ppsk_device_alloc", "ppsk_device_free", "ppsk_copy_to_host",
+    "ppsk_last_timing", "ppsk_synth_sketches",
 ]
 
 
@@ -60,13 +61,18 @@ def lib():
     L.ppsk_set_size.restype = _i64
     L.ppsk_dense.argtypes = [_p, _p, _i32, _i32, _p, _c.c_double, _i32, _p, _i32,
                              _c.POINTER(_u64), _c.POINTER(_i64)]
-    L.ppsk_knn.argtypes = [_p, _i32, _i32, _p, _c.c_double, _i32, _i32, _p, _p, _p,
+    L.ppsk_knn.argtypes = [_p, _i32, _i32, _p, _c.c_double, _i32, _i32, _i32, _p, _p, _p,
                            _c.POINTER(_u64)]
     L.ppsk_threshold.argtypes = [_p, _i32, _i32, _p, _c.c_double, _i32, _c.c_float,
                                  _c.POINTER(_p), _c.POINTER(_p), _c.POINTER(_p),
                                  _c.POINTER(_i64), _c.POINTER(_u64)]
+    L.ppsk_self_pass.argtypes = [_p, _i32, _i32, _p, _c.c_double, _p, _i32, _i32, _p, _p, _p,
+                                 _c.c_float, _i32, _c.POINTER(_p), _c.POINTER(_p),
+                                 _c.POINTER(_p), _c.POINTER(_i64), _c.POINTER(_u64)]
     L.ppsk_free_host.argtypes = [_p]
     L.ppsk_free_host.restype = None
+    L.ppsk_host_alloc.argtypes = [_c.c_size_t, _c.POINTER(_p)]
+    L.ppsk_host_free.argtypes = [_p]
     L.ppsk_device_alloc.argtypes = [_c.c_int, _c.c_size_t, _c.POINTER(_p)]
     L.ppsk_device_free.argtypes = [_p]
     L.ppsk_copy_to_host.argtypes = [_p, _p, _c.c_size_t]
@@ -210,7 +216,7 @@ def dense_device(ref, query=None, random_mode=RANDOM_NONE, table=None, tol=None,
     return dptr, nf.value
 
 
-def knn(ref, kNN, dist_col=0, random_mode=RANDOM_NONE, table=None, tol=None):
+def knn(ref, kNN, dist_col=0, random_mode=RANDOM_NONE, table=None, tol=None, jaccard=False):
     """Self kNN: (i_vec, j_vec, dists, n_fail)."""
     iv = np.zeros(ref.n * kNN, np.int64)
     jv = np.zeros(ref.n * kNN, np.int64)
@@ -218,9 +224,76 @@ def knn(ref, kNN, dist_col=0, random_mode=RANDOM_NONE, table=None, tol=None):
     mode, ncl, tab = _random_args(random_mode, table)
     nf = _u64()
     tol = default_tolerance(ref.s64) if tol is None else tol
-    _check(lib().ppsk_knn(ref.handle, mode, ncl, _ptr(tab), tol, kNN, dist_col, _ptr(iv),
-                          _ptr(jv), _ptr(dv), _c.byref(nf)), allow=(ERR_FIT,))
+    _check(lib().ppsk_knn(ref.handle, mode, ncl, _ptr(tab), tol, int(bool(jaccard)), kNN,
+                          dist_col, _ptr(iv), _ptr(jv), _ptr(dv), _c.byref(nf)),
+           allow=(ERR_FIT,))
     return iv, jv, dv, nf.value
+
+
+def _take_host_lists(pi, pj, pd, m):
+    """Copy library-allocated (i, j, d) host arrays of m entries, then free them."""
+    try:
+        iv = np.ctypeslib.as_array(_c.cast(pi, _c.POINTER(_i64)), (max(m, 1),))[:m].copy()
+        jv = np.ctypeslib.as_array(_c.cast(pj, _c.POINTER(_i64)), (max(m, 1),))[:m].copy()
+        dv = np.ctypeslib.as_array(_c.cast(pd, _c.POINTER(_c.c_float)), (max(m, 1),))[:m].copy()
+    finally:
+        for x in (pi, pj, pd):
+            if x:
+                lib().ppsk_free_host(x)
+    return iv, jv, dv
+
+
+class PinnedBuffer:
+    """Page-locked host memory from ppsk_host_alloc, viewed as a numpy array."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dtype.itemsize
+        self.ptr = _p()
+        _check(lib().ppsk_host_alloc(max(nbytes, 1), _c.byref(self.ptr)))
+        buf = (_c.c_char * max(nbytes, 1)).from_address(self.ptr.value)
+        self.array = np.frombuffer(buf, dtype=np.uint8, count=nbytes).view(dtype).reshape(shape)
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            _check(lib().ppsk_host_free(self.ptr))
+            self.ptr = _p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def self_pass(ref, dense_out=None, kNN=0, knn_col=0, cutoff=0.0, thr_col=0,
+              random_mode=RANDOM_NONE, table=None, tol=None):
+    """One self pass: condensed (core, acc) into dense_out (a float32
+    [n(n-1)/2, 2] array, e.g. PinnedBuffer.array, or None), kNN lists and
+    threshold lists.  Returns (knn (i, j, d) or None, thr (i, j, d) or None,
+    n_fail)."""
+    mode, ncl, tab = _random_args(random_mode, table)
+    tol = default_tolerance(ref.s64) if tol is None else tol
+    if dense_out is not None:
+        rows = ref.n * (ref.n - 1) // 2
+        if dense_out.dtype != np.float32 or dense_out.size != rows * 2 or \
+                not dense_out.flags["C_CONTIGUOUS"]:
+            raise PpskError(ERR_ARG, "dense_out must be contiguous float32 [n(n-1)/2, 2]")
+    ki = kj = kd = None
+    if kNN > 0:
+        ki = np.zeros(ref.n * kNN, np.int64)
+        kj = np.zeros(ref.n * kNN, np.int64)
+        kd = np.zeros(ref.n * kNN, np.float32)
+    pi, pj, pd = _p(), _p(), _p()
+    n = _i64()
+    nf = _u64()
+    _check(lib().ppsk_self_pass(ref.handle, mode, ncl, _ptr(tab), tol, _ptr(dense_out), kNN,
+                                knn_col, _ptr(ki), _ptr(kj), _ptr(kd), cutoff, thr_col,
+                                _c.byref(pi), _c.byref(pj), _c.byref(pd), _c.byref(n),
+                                _c.byref(nf)), allow=(ERR_FIT,))
+    thr = _take_host_lists(pi, pj, pd, n.value) if cutoff > 0 else None
+    return (None if kNN == 0 else (ki, kj, kd)), thr, nf.value
 
 
 def threshold(ref, cutoff, dist_col=0, random_mode=RANDOM_NONE, table=None, tol=None):
@@ -233,15 +306,7 @@ def threshold(ref, cutoff, dist_col=0, random_mode=RANDOM_NONE, table=None, tol=
     _check(lib().ppsk_threshold(ref.handle, mode, ncl, _ptr(tab), tol, dist_col, cutoff,
                                 _c.byref(pi), _c.byref(pj), _c.byref(pd), _c.byref(n),
                                 _c.byref(nf)), allow=(ERR_FIT,))
-    m = n.value
-    try:
-        iv = np.ctypeslib.as_array(_c.cast(pi, _c.POINTER(_i64)), (max(m, 1),))[:m].copy()
-        jv = np.ctypeslib.as_array(_c.cast(pj, _c.POINTER(_i64)), (max(m, 1),))[:m].copy()
-        dv = np.ctypeslib.as_array(_c.cast(pd, _c.POINTER(_c.c_float)), (max(m, 1),))[:m].copy()
-    finally:
-        for x in (pi, pj, pd):
-            if x:
-                lib().ppsk_free_host(x)
+    iv, jv, dv = _take_host_lists(pi, pj, pd, n.value)
     return iv, jv, dv, nf.value
 
 

@@ -38,3 +38,14 @@ def mc_table(klist, n_clusters=3, seed=5, symmetric=True, zero_from=None):
         lt = np.array([k < zero_from for k in klist])
         t[~lt] = 0
     return t, lt
+
+
+def max_abs_diff(a, b):
+    """max |a - b| over finite entries; infinities must coincide exactly."""
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    assert a.shape == b.shape
+    inf_a, inf_b = np.isinf(a), np.isinf(b)
+    assert np.array_equal(inf_a, inf_b) and np.array_equal(a[inf_a], b[inf_b])
+    fin = ~inf_a
+    return float(np.abs(a[fin] - b[fin]).max()) if fin.any() else 0.0

@@ -10,7 +10,7 @@ import pytest
 
 import oracle as O
 import ppsk_hip as H
-from helpers import KLIST8, mc_table, synth, with_duplicates
+from helpers import KLIST8, mc_table, synth, with_duplicates, max_abs_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -93,7 +93,7 @@ def test_knn_matches_heap_order(knn, col):
     assert np.array_equal(iv, ri)
     # distances agree to the core/acc tolerance; indices exactly where the
     # distance is not within tolerance of a tie
-    assert np.abs(dv - rd).max() <= CORE_ACC_TOL
+    assert max_abs_diff(dv, rd) <= CORE_ACC_TOL
     assert np.array_equal(jv, rj)
 
 
@@ -120,4 +120,68 @@ def test_threshold(cutoff, col):
     ri, rj, rd = O.sparsify_threshold(square, cutoff)
     assert np.array_equal(iv, ri)
     assert np.array_equal(jv, rj)
-    assert np.abs(dv - rd).max() <= CORE_ACC_TOL
+    assert max_abs_diff(dv, rd) <= CORE_ACC_TOL
+
+
+@pytest.mark.parametrize("chunk_bytes", [None, "1"])
+def test_self_pass_equals_separate_calls(monkeypatch, chunk_bytes):
+    # chunk_bytes "1": one row tile per launch, so the host copies of the
+    # dense rows run block by block behind the computation
+    if chunk_bytes:
+        monkeypatch.setenv("PPSK_CHUNK_BYTES", chunk_bytes)
+    raw, le, _ = synth(300)
+    raw, le = with_duplicates(raw, le, [(3, 10), (57, 58)])
+    n = raw.shape[0]
+    with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
+        d_sep, _, _ = H.dense(s, random_mode=H.RANDOM_EQUAL_BASE)
+        k_sep = H.knn(s, 10, dist_col=1, random_mode=H.RANDOM_EQUAL_BASE)
+        t_sep = H.threshold(s, 0.02, dist_col=0, random_mode=H.RANDOM_EQUAL_BASE)
+        _, launches, _ = H.last_timing()
+        with H.PinnedBuffer((n * (n - 1) // 2, 2), np.float32) as pin:
+            kn, th, nf = H.self_pass(s, pin.array, kNN=10, knn_col=1, cutoff=0.02, thr_col=0,
+                                     random_mode=H.RANDOM_EQUAL_BASE)
+            dense = pin.array.copy()
+        _, launches_pass, _ = H.last_timing()
+    if chunk_bytes:
+        assert launches_pass > 1
+    assert nf == 0
+    assert np.array_equal(dense.view(np.uint32), d_sep.view(np.uint32))
+    for a, b in zip(kn, k_sep[:3]):
+        assert np.array_equal(a, b)
+    for a, b in zip(th, t_sep[:3]):
+        assert np.array_equal(a, b)
+    ref, _, _, _ = O.query_db(raw, KLIST8, 156, 14, le, random=O.Random(1, True))
+    assert np.abs(dense - ref).max() <= CORE_ACC_TOL
+
+
+def test_dense_streamed_chunks_match_single_launch(monkeypatch):
+    raw, le, _ = synth(300)
+    with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
+        one, _, _ = H.dense(s, out="counts")
+        q = H.SketchSet(raw[:50].copy(), KLIST8, 156, lengths=le[:50].copy())
+        with q:
+            qone, _, _ = H.dense(s, q, random_mode=H.RANDOM_EQUAL_BASE)
+            monkeypatch.setenv("PPSK_CHUNK_BYTES", "1")
+            many, _, _ = H.dense(s, out="counts")
+            _, launches, _ = H.last_timing()
+            qmany, _, _ = H.dense(s, q, random_mode=H.RANDOM_EQUAL_BASE)
+    assert launches > 1
+    assert np.array_equal(one, many)
+    assert np.array_equal(qone.view(np.uint32), qmany.view(np.uint32))
+
+
+def test_knn_asymmetric_table_band_path():
+    # an asymmetric Monte Carlo table makes d(i, j) != d(j, i): kNN must
+    # come from full square rows, not from the triangle
+    raw, le, _ = synth(200)
+    table, lt = mc_table(KLIST8, symmetric=False, zero_from=27)
+    cl = (np.arange(200) * 7 % 3).astype(np.uint16)
+    with H.SketchSet(raw, KLIST8, 156, lengths=le, clusters=cl) as s:
+        iv, jv, dv, nf = H.knn(s, 6, dist_col=0, random_mode=H.RANDOM_MC, table=table)
+        with pytest.raises(H.PpskError):
+            H.self_pass(s, None, kNN=6, random_mode=H.RANDOM_MC, table=table)
+    ri, rj, rd, rnf = O.query_db_sparse(raw, KLIST8, 156, 14, le, 6, ref_cluster=cl,
+                                        random=O.Random(2, True, table, lt))
+    assert nf == rnf == 0
+    assert np.array_equal(iv, ri) and np.array_equal(jv, rj)
+    assert max_abs_diff(dv, rd) <= CORE_ACC_TOL
