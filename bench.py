@@ -43,9 +43,42 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=N_SAMPLES)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-sample", type=int, default=4096,
+    ap.add_argument("--cpu-sample", type=int, default=8192,
                     help="samples of the CPU baseline's all-vs-all sample")
+    ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
+                    help="run one plain copy kernel of BYTES and exit (HBM counter calibration)")
     return ap.parse_args()
+
+
+def _bench_lib():
+    import ctypes
+    L = ctypes.CDLL(os.path.join(ROOT, "pp-sketchlib_amd", "libppsk_bench.so"))
+    L.ppsk_bench_copy.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_double)]
+    L.ppsk_bench_valu_peak.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]
+    return L
+
+
+def copy_only(nbytes):
+    """One copy of nbytes (read nbytes, write nbytes) for counter calibration."""
+    import ctypes
+    ms = ctypes.c_double()
+    rc = _bench_lib().ppsk_bench_copy(0, nbytes, 1, ctypes.byref(ms))
+    if rc:
+        raise RuntimeError("copy kernel failed")
+    print(json.dumps({"copy_bytes": nbytes, "ms": ms.value,
+                      "GB_per_s": 2 * nbytes / ms.value / 1e6}), flush=True)
+
+
+def measured_valu_peak():
+    """Chained v_bitop3_b32 microbenchmark: int32 VALU ops/s this GPU sustains."""
+    import ctypes
+    ops, ms = ctypes.c_double(), ctypes.c_double()
+    if _bench_lib().ppsk_bench_valu_peak(0, 16384, 2000, ctypes.byref(ops), ctypes.byref(ms)):
+        return None
+    return ops.value / 1e12
 
 
 def traffic_from_profile():
@@ -83,8 +116,11 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              "(the distance path shards by tile blocks, see DESIGN.md)", file=sys.stderr)
+              "(the distance path shards by tile blocks across GPUs; only one GPU is measured here)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_only:
+        copy_only(a.copy_only)
+        return
     import ppsk_hip as H
 
     t0 = time.time()
@@ -123,7 +159,11 @@ def main():
         "kernel": "dist_tiles",
         "kernel_ms": round(per_launch_ms, 3),
         "ops_per_pair": OPS_PER_PAIR,
+        "peak_measured": None,
     }
+    peak_meas = measured_valu_peak()
+    if peak_meas:
+        roofline["peak_measured"] = round(peak_meas, 3)
     cpu = None
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -137,23 +137,38 @@ int check_compatible(const ppsk_set* a, const ppsk_set* b) {
   return PPSK_OK;
 }
 
-// Tile lists for a tile shape of tr rows x tc columns.  Self: upper-triangle
-// tiles (at least one pair with i < j).  Full square / query: every tile.
-// Row-major order (consecutive tiles share their row block).  row_begin must
-// be a multiple of tr.
+// Tile list for a tile shape of tr rows x tc columns over row tiles
+// [row_begin / tr, ceil(row_end / tr)) (row_begin a multiple of tr).  Self
+// mode keeps upper-triangle tiles only (at least one pair with i < j).
+//
+// Order: row tiles are taken in groups of `group`; inside a group the tiles
+// run column by column, each column visiting every row tile of the group.
+// Consecutive workgroups of a launch land on the same XCD eight apart, so the
+// workgroups an XCD holds at once cover a compact block of `group` row tiles
+// by a few column tiles: each column panel and row panel is fetched into that
+// XCD's L2 once and reused by the whole block.  group_start receives the
+// first tile of every group plus the end of the list.
 std::vector<ppsk::Tile> make_tiles(int64_t n_rows, int64_t n_cols, int tr, int tc, bool upper,
-                                   int64_t row_begin, int64_t row_end) {
+                                   int64_t row_begin, int64_t row_end, int group,
+                                   std::vector<size_t>* group_start) {
   std::vector<ppsk::Tile> t;
   const int64_t nbr = (n_rows + tr - 1) / tr;
   const int64_t nct = (n_cols + tc - 1) / tc;
   const int64_t b0 = row_begin / tr;
   const int64_t b1 = std::min(nbr, (row_end + tr - 1) / tr);
-  for (int64_t bi = b0; bi < b1; ++bi) {
+  group_start->clear();
+  for (int64_t g0 = b0; g0 < b1; g0 += group) {
+    const int64_t g1 = std::min(b1, g0 + group);
+    const size_t before = t.size();
     for (int64_t ct = 0; ct < nct; ++ct) {
-      if (upper && (ct + 1) * tc - 1 <= bi * tr) continue;
-      t.push_back({(int32_t)bi, (int32_t)ct});
+      for (int64_t bi = g0; bi < g1; ++bi) {
+        if (upper && (ct + 1) * tc - 1 <= bi * tr) continue;
+        t.push_back({(int32_t)bi, (int32_t)ct});
+      }
     }
+    if (t.size() > before) group_start->push_back(before);
   }
+  group_start->push_back(t.size());
   return t;
 }
 
@@ -184,6 +199,28 @@ int check_random(const ppsk_set* rows, const ppsk_set* cols, const RandomIn& rnd
   return PPSK_OK;
 }
 
+// Row tiles per scheduling group (make_tiles); PPSK_TILE_GROUP overrides.
+int tile_group() {
+  static int g = -1;
+  if (g < 0) {
+    const char* e = std::getenv("PPSK_TILE_GROUP");
+    g = e ? std::atoi(e) : 8;
+    if (g < 1) g = 1;
+  }
+  return g;
+}
+
+// Tile kernel configuration: PPSK_TILE_VARIANT overrides the default.
+int default_variant() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("PPSK_TILE_VARIANT");
+    v = e ? std::atoi(e) : ppsk::DEFAULT_VARIANT;
+    if (!ppsk::variant_valid(v)) v = ppsk::DEFAULT_VARIANT;
+  }
+  return v;
+}
+
 // Timing of the tile-kernel launches of one call: per-chunk events on the
 // launch stream, summed once the call has finished.
 struct Timing {
@@ -210,40 +247,40 @@ int64_t out_row_start(bool self, int64_t r, int64_t n_rows, int64_t n_cols) {
   return n * r - ((r * (r + 1)) >> 1);
 }
 
-// Runs the tile kernel over `tiles` (row-major by row tile) writing dense
-// output into the device buffer p.out (row offset 0).  Chunks are cut at row
-// tile boundaries so that each fits the per-tile count scratch; when host_out
-// is set, the finished output rows of each chunk are copied to host_out on
-// the copy stream while the following chunks compute.  Returns after both
-// streams have drained.
-int run_dense(DeviceCtx* c, ppsk::DistParams p, int variant, const std::vector<ppsk::Tile>& tiles,
-              bool self, size_t elem, void* host_out, Timing* tm) {
-  if (tiles.empty()) return PPSK_OK;
+// Runs the tile kernel over the row tiles covering sample rows
+// [row_begin, row_end) (row_begin a multiple of the tile height) against all
+// columns, writing dense output into the device buffer p.out (row offset 0).
+// Launches are cut at group boundaries (whole row tiles) so that each fits
+// the per-tile count scratch; when host_out is set, the finished output rows
+// of each launch are copied to host_out on the copy stream while the
+// following launches compute.  Returns after both streams have drained.
+int run_dense(DeviceCtx* c, ppsk::DistParams p, int variant, bool upper, int64_t row_begin,
+              int64_t row_end, bool self, size_t elem, void* host_out, Timing* tm) {
   const int tr = ppsk::variant_tile_rows(variant);
-  const size_t per_tile =
-      (size_t)p.n_k * tr * ppsk::variant_tile_cols(variant) * sizeof(uint16_t);
+  const int tc = ppsk::variant_tile_cols(variant);
+  const size_t per_tile = (size_t)p.n_k * tr * tc * sizeof(uint16_t);
   // count scratch per launch; PPSK_CHUNK_BYTES lowers it (more, smaller
-  // chunks: finer-grained streaming of host output)
+  // launches: finer-grained streaming of host output)
   size_t budget = (size_t)1 << 30;
   if (const char* env = std::getenv("PPSK_CHUNK_BYTES")) {
     const long long v = std::atoll(env);
     if (v > 0) budget = (size_t)v;
   }
   const size_t max_tiles = std::max<size_t>(1, budget / per_tile);
-  // chunk boundaries at row-tile changes
+  const int64_t nct = (p.n_cols + tc - 1) / tc;
+  int group = tile_group();
+  group = (int)std::max<int64_t>(1, std::min<int64_t>(group, (int64_t)max_tiles / nct));
+  std::vector<size_t> gs;
+  const std::vector<ppsk::Tile> tiles =
+      make_tiles(p.n_rows, p.n_cols, tr, tc, upper, row_begin, row_end, group, &gs);
+  if (tiles.empty()) return PPSK_OK;
+  // launches: runs of whole groups within the budget (at least one group)
   std::vector<std::pair<size_t, size_t>> chunks;
-  size_t t0 = 0;
-  while (t0 < tiles.size()) {
-    size_t t1 = t0;
-    while (t1 < tiles.size()) {
-      size_t e = t1;
-      while (e < tiles.size() && tiles[e].bi == tiles[t1].bi) ++e;  // one row tile
-      if (t1 > t0 && e - t0 > max_tiles) break;
-      t1 = e;
-      if (t1 - t0 >= max_tiles) break;
-    }
-    chunks.emplace_back(t0, t1);
-    t0 = t1;
+  for (size_t g = 0; g + 1 < gs.size();) {
+    size_t h = g + 1;
+    while (h + 1 < gs.size() && gs[h + 1] - gs[g] <= max_tiles) ++h;
+    chunks.emplace_back(gs[g], gs[h]);
+    g = h;
   }
   size_t most = 0;
   for (auto& ch : chunks) most = std::max(most, ch.second - ch.first);
@@ -278,8 +315,13 @@ int run_dense(DeviceCtx* c, ppsk::DistParams p, int variant, const std::vector<p
   // block this thread until it completes, which must not hold back launches.
   for (size_t ci = 0; host_out && ci < chunks.size(); ++ci) {
     const size_t a = chunks[ci].first, b = chunks[ci].second;
-    const int64_t r0 = (int64_t)tiles[a].bi * tr;
-    const int64_t r1 = (int64_t)(tiles[b - 1].bi + 1) * tr;
+    int32_t bmin = tiles[a].bi, bmax = tiles[a].bi;
+    for (size_t x = a; x < b; ++x) {
+      bmin = std::min(bmin, tiles[x].bi);
+      bmax = std::max(bmax, tiles[x].bi);
+    }
+    const int64_t r0 = (int64_t)bmin * tr;
+    const int64_t r1 = (int64_t)(bmax + 1) * tr;
     const int64_t o0 = out_row_start(self, r0, p.n_rows, p.n_cols);
     const int64_t o1 = out_row_start(self, r1, p.n_rows, p.n_cols);
     if (o1 > o0) {
@@ -343,23 +385,6 @@ int read_fail(DeviceCtx* c, unsigned long long* cnt, unsigned long long* first) 
   return PPSK_OK;
 }
 
-// Tile kernel configuration: PPSK_TILE_VARIANT overrides the default.
-int default_variant() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = std::getenv("PPSK_TILE_VARIANT");
-    v = e ? std::atoi(e) : ppsk::DEFAULT_VARIANT;
-    if (!ppsk::variant_valid(v)) v = ppsk::DEFAULT_VARIANT;
-  }
-  return v;
-}
-
-std::vector<ppsk::Tile> variant_tiles(int variant, int64_t n_rows, int64_t n_cols, bool upper,
-                                      int64_t row_begin, int64_t row_end) {
-  return make_tiles(n_rows, n_cols, ppsk::variant_tile_rows(variant),
-                    ppsk::variant_tile_cols(variant), upper, row_begin, row_end);
-}
-
 bool table_symmetric(const RandomIn& rnd, int n_k) {
   if (rnd.mode != PPSK_RANDOM_MC) return true;
   const int C = rnd.n_clusters;
@@ -395,8 +420,7 @@ int dense_pass(DeviceCtx* c, const ppsk_set* rows, const ppsk_set* cols, bool se
   int rc = reset_fail(c);
   if (rc == PPSK_OK) {
     const int variant = default_variant();
-    auto tiles = variant_tiles(variant, rows->n, cols->n, self, 0, rows->n);
-    rc = run_dense(c, p, variant, tiles, self, elem, host_out, tm);
+    rc = run_dense(c, p, variant, self, 0, rows->n, self, elem, host_out, tm);
   }
   if (dtab) hipFree(dtab);
   if (rc) return rc;
@@ -731,8 +755,7 @@ int ppsk_knn(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float
       p.mc_ref_is_col = 0;
       p.fail_count = c->fail;
       p.fail_first = c->fail + 1;
-      auto tiles = variant_tiles(variant, n, n, false, r0, r1);
-      rc = run_dense(c, p, variant, tiles, true, sizeof(float2), nullptr, &tm);
+      rc = run_dense(c, p, variant, false, r0, r1, true, sizeof(float2), nullptr, &tm);
       if (rc) break;
       e = ppsk::launch_knn_select((const float*)c->out, r0, r1 - r0, n, dist_col, kNN, d_i, d_j,
                                   d_d, c->stream);
