@@ -86,6 +86,16 @@ int ppsk_set_upload(ppsk_set *set, int64_t first, int64_t count,
 int ppsk_set_meta(ppsk_set *set, const uint64_t *lengths, int32_t use_rc,
                   const uint16_t *clusters);
 
+/*
+ * A read-only view of the first count samples of a set (no copy): a drop-in
+ * ppsk_set for every call, sharing the parent's device buffers and
+ * metadata.  Free it with ppsk_set_free (which then frees only the view)
+ * before the parent.  Lets many jobs query prefixes of one resident
+ * database, as the reference's sample lists select sub-databases
+ * (src/api.cpp:436-516).
+ */
+int ppsk_set_prefix(ppsk_set *parent, int64_t count, ppsk_set **view);
+
 int ppsk_set_free(ppsk_set *set);
 
 int64_t ppsk_set_size(const ppsk_set *set);
@@ -172,6 +182,112 @@ int ppsk_host_free(void *ptr);
 int ppsk_device_alloc(int device_id, size_t bytes, void **ptr);
 int ppsk_device_free(void *ptr);
 int ppsk_copy_to_host(void *dst, const void *device_src, size_t bytes);
+
+/*
+ * Batched query-vs-ref jobs in one persistent launch.  Each job is a
+ * ppsk_dense call in query mode (rows = query samples, query-major output,
+ * r1 = query) with its own sets, k-mer lengths, sketch size, random
+ * correction, tolerance and device output buffer (ppsk_device_alloc, sized
+ * as for ppsk_dense).  The reference runs such jobs one queryDatabase /
+ * query_db_cuda call at a time (src/gpu/gpu_api.cpp:352-452); results here
+ * are identical to those single calls.
+ */
+typedef struct ppsk_job {
+  ppsk_set *ref;        /* columns */
+  ppsk_set *query;      /* rows */
+  int32_t random_mode;  /* PPSK_RANDOM_* */
+  int32_t n_clusters;   /* PPSK_RANDOM_MC: table [n_k][C][C] */
+  const float *mc_table;
+  double tolerance;
+  int32_t out_kind;     /* PPSK_OUT_CORE_ACC / JACCARD / COUNTS */
+  void *out;            /* device memory */
+} ppsk_job;
+
+typedef struct ppsk_batch ppsk_batch;
+
+/*
+ * Plan a batch: validates every job, uploads per-job parameters and tables.
+ * The sets must stay alive (and unchanged) while the batch exists.
+ */
+int ppsk_batch_create(int device_id, const ppsk_job *jobs, int32_t n_jobs,
+                      ppsk_batch **out);
+
+/*
+ * Run every job: one graph launch (queue reset + persistent kernel), replayed
+ * on later runs.  variant: tile shape (2: 64 x 256, 6: 32 x 256) or 0 for the
+ * autotuner, which on the first run of a batch with an unseen job-shape
+ * signature times each shape on the batch itself and keeps the fastest for
+ * the process.  n_fail (may be NULL): per-job count of pairs whose regression
+ * failed (PPSK_ERR_FIT is returned after all output is written).
+ * used_variant / kernel_ms (may be NULL): shape used and its launch time.
+ */
+int ppsk_batch_run(ppsk_batch *batch, int32_t variant, uint64_t *n_fail,
+                   int32_t *used_variant, double *kernel_ms);
+
+/* Autotuner result: chosen shape and per-shape time (index = shape, 0..6). */
+int ppsk_batch_tuning(const ppsk_batch *batch, int32_t *variant,
+                      double *ms_per_variant);
+
+int ppsk_batch_free(ppsk_batch *batch);
+
+/*
+ * Serving: a resident reference set answers query batches that arrive at any
+ * time.  Each request returns, per query, the kNN nearest references by
+ * column knn_col (0 core, 1 accessory; the heap order of ppsk_knn, no sample
+ * excluded) and every reference with column thr_col below cutoff (row-major,
+ * as ppsk_threshold), with r1 = the query (query mode of ppsk_dense).  The
+ * reference's closest interface is queryDatabase followed by a sparse
+ * extraction, one call per query batch (src/sketchlib_bindings.cpp:89-183).
+ *
+ * Continuous batching: n_slots workers, each with its own stream, take as
+ * many queued requests (oldest first) as fit one step of at most max_rows
+ * query samples whose device memory fits the serving arena (arena_bytes;
+ * 0 = free HBM less 4 GiB); requests arriving while other steps run join the
+ * next step that has room, and steps of different workers overlap on the
+ * GPU.  A request's results do not depend on the step it joins.
+ * random_mode: PPSK_RANDOM_NONE or PPSK_RANDOM_EQUAL_BASE (query lengths
+ * passed at submit).  The reference set must outlive the server.
+ */
+typedef struct ppsk_server ppsk_server;
+
+typedef struct ppsk_result {
+  int64_t n_queries;
+  int64_t knn_n; /* n_queries * kNN; i = query index within the request */
+  int64_t *knn_i, *knn_j;
+  float *knn_d;
+  int64_t thr_n;
+  int64_t *thr_i, *thr_j;
+  float *thr_d;
+  uint64_t n_fail;  /* pairs whose regression failed (distances 0, 0) */
+  double t_submit, t_start, t_done; /* seconds since server creation */
+  int64_t step_id;   /* step that served the request */
+  int64_t step_rows; /* query samples in that step */
+  void *owner;       /* library-internal */
+} ppsk_result;
+
+int ppsk_server_create(ppsk_set *ref, int32_t random_mode, double tolerance,
+                       int32_t kNN, int32_t knn_col, float cutoff,
+                       int32_t thr_col, int32_t max_rows, int32_t n_slots,
+                       size_t arena_bytes, ppsk_server **out);
+
+/* Queue a request: raw [n_queries][n_k][s64 * 14] in the reference layout,
+ * lengths [n_queries] (equal-base mode).  Copies the inputs; never blocks on
+ * the GPU. */
+int ppsk_server_submit(ppsk_server *server, int64_t n_queries,
+                       const uint64_t *raw, const uint64_t *lengths,
+                       int64_t *ticket);
+
+/* Block until the request is served; release the result with
+ * ppsk_result_free.  Each ticket can be collected once. */
+int ppsk_server_wait(ppsk_server *server, int64_t ticket, ppsk_result **out);
+void ppsk_result_free(ppsk_result *result);
+
+int ppsk_server_stats(ppsk_server *server, int64_t *steps, int64_t *rows,
+                      int64_t *requests, int32_t *peak_inflight,
+                      size_t *peak_arena_bytes, double *gpu_ms);
+
+/* Stops the workers (queued requests fail); no wait may be in progress. */
+int ppsk_server_free(ppsk_server *server);
 
 /*
  * Timing of the last dense/kNN/threshold call on this thread, measured with

@@ -416,6 +416,37 @@ void orc_knn_from_square(const float *square, long long n, int kNN, long *i_vec,
   }
 }
 
+/*
+ * The same heap over an explicit set of m dense rows [m][n]: row r scans
+ * columns 0..n-1 with column row_ids[r] treated as +inf (row_ids[r] < 0: no
+ * column excluded, as for a query row against a separate reference set).
+ * i_vec receives row_ids[r] (or r when it is negative).
+ */
+void orc_knn_rows(const float *rows, const long *row_ids, long long m, long long n, int kNN,
+                  long *i_vec, long *j_vec, float *dists, int num_threads) {
+#pragma omp parallel for schedule(static) num_threads(num_threads)
+  for (long long r = 0; r < m; r++) {
+    std::priority_queue<OrcSparseDist> min_dists;
+    const long self = row_ids[r];
+    for (long long j = 0; j < n; j++) {
+      float row_dist = j == self ? std::numeric_limits<float>::infinity() : rows[r * n + j];
+      if ((long long)min_dists.size() < kNN || row_dist < min_dists.top().dist) {
+        OrcSparseDist nm = {row_dist, (long)j};
+        min_dists.push(nm);
+        if ((long long)min_dists.size() > kNN) min_dists.pop();
+      }
+    }
+    const long long offset = r * (long long)kNN;
+    for (int k = 0; k < kNN; ++k) i_vec[offset + k] = self >= 0 ? self : (long)r;
+    for (int k = kNN - 1; k >= 0; --k) {
+      OrcSparseDist e = min_dists.top();
+      j_vec[offset + k] = e.j;
+      dists[offset + k] = e.dist;
+      min_dists.pop();
+    }
+  }
+}
+
 /* ---------------------------------------------------------------------------
  * Index math -- dist/matrix_idx.hpp:13-50.
  * ------------------------------------------------------------------------- */

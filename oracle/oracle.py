@@ -57,6 +57,8 @@ def _load():
                                         _c.c_int, _p, _p, _p]
     lib.orc_knn_from_square.restype = None
     lib.orc_knn_from_square.argtypes = [_p, _ll, _c.c_int, _p, _p, _p, _c.c_int]
+    lib.orc_knn_rows.restype = None
+    lib.orc_knn_rows.argtypes = [_p, _p, _ll, _ll, _c.c_int, _p, _p, _p, _c.c_int]
     lib.orc_square_to_condensed.restype = _ll
     lib.orc_square_to_condensed.argtypes = [_c.c_long, _c.c_long, _c.c_long]
     lib.orc_calc_row_idx.restype = _c.c_long
@@ -246,6 +248,21 @@ def knn_from_square(square, kNN, num_threads=None):
     dv = np.zeros(n * kNN, np.float32)
     L.orc_knn_from_square(_ptr(sq), n, int(kNN), _ptr(iv), _ptr(jv), _ptr(dv),
                           num_threads or L.orc_max_threads())
+    return iv, jv, dv
+
+
+def knn_rows(rows, row_ids, kNN, num_threads=None):
+    """The same heap over dense rows [m][n]; column row_ids[r] counts as +inf
+    (row_ids[r] < 0: none excluded).  Returns (i_vec, j_vec, dists)."""
+    L = lib()
+    d = np.ascontiguousarray(rows, np.float32)
+    m, n = d.shape
+    ids = np.ascontiguousarray(row_ids, np.int64)
+    iv = np.zeros(m * kNN, np.int64)
+    jv = np.zeros(m * kNN, np.int64)
+    dv = np.zeros(m * kNN, np.float32)
+    L.orc_knn_rows(_ptr(d), _ptr(ids), m, n, int(kNN), _ptr(iv), _ptr(jv), _ptr(dv),
+                   num_threads or L.orc_max_threads())
     return iv, jv, dv
 
 

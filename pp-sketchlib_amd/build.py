@@ -21,7 +21,7 @@ HDF5_PREFIX = os.environ.get("PPSK_HDF5_PREFIX", "/opt/conda")
 
 LIB = os.path.join(HERE, "libppsk.so")
 
-HIP_SOURCES = ["dist_kernels.hip", "select_kernels.hip", "ppsk_capi.cpp"]
+HIP_SOURCES = ["dist_kernels.hip", "select_kernels.hip", "ppsk_capi.cpp", "batch.cpp", "serve.cpp"]
 HOST_SOURCES = ["synth.cpp"]
 
 HIP_FLAGS = [

@@ -1,0 +1,113 @@
+// Shared internals of the C-ABI layer (ppsk_capi.cpp, batch.cpp, serve.cpp):
+// the sketch-set handle, per-device contexts, argument checks and the tile
+// scheduling helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ppsk.h"
+#include "ppsk_internal.h"
+
+struct ppsk_set {
+  int device;
+  int64_t n;
+  int32_t n_k;
+  std::vector<int32_t> klist;
+  int32_t s64;
+  int32_t bbits;
+  int32_t nb;
+  uint32_t* stripes = nullptr;  // [n_k][nb][s64][28][128]
+  double* jv = nullptr;         // [n][n_k]
+  uint16_t* clusters = nullptr; // [n]
+  bool has_lengths = false;
+  int32_t use_rc = 1;
+  int32_t max_cluster = -1;     // largest cluster id uploaded
+  bool view = false;            // prefix view: shares the parent's buffers
+};
+
+namespace ppsk {
+namespace capi {
+
+extern thread_local std::string g_last_error;
+extern thread_local double g_last_ms;
+extern thread_local int64_t g_last_launches;
+extern thread_local int64_t g_last_items;
+
+// Records msg as this thread's last error and returns code.
+int fail(int code, const std::string& msg);
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      return fail(_e == hipErrorOutOfMemory ? PPSK_ERR_MEMORY : PPSK_ERR_HIP,      \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));              \
+    }                                                                              \
+  } while (0)
+
+// Per-device state: a launch stream, a copy stream, cached workspaces grown on
+// demand.
+struct DeviceCtx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  void* counts = nullptr;
+  size_t counts_bytes = 0;
+  uint32_t* tile_done = nullptr;
+  size_t tile_done_n = 0;
+  ppsk::Tile* tiles = nullptr;
+  size_t tiles_n = 0;
+  unsigned long long* fail = nullptr;  // [0] count, [1] first row
+  void* out = nullptr;
+  size_t out_bytes = 0;
+  hipStream_t copy_stream = nullptr;  // device -> host copies of finished rows
+  std::vector<hipEvent_t> events;     // per-chunk timing / copy dependencies
+  std::mutex mu;
+};
+
+// Context of a device (created on first use) made current.
+int get_ctx(int device, DeviceCtx** out);
+// Device buffer of at least `need` bytes (reallocated, contents lost).
+int grow(void** ptr, size_t* have, size_t need);
+// Tile completion counters, zero on return.
+int grow_tile_done(DeviceCtx* c, size_t n);
+
+int check_set(const ppsk_set* s, const char* what);
+int check_compatible(const ppsk_set* a, const ppsk_set* b);
+
+std::vector<ppsk::Tile> make_tiles(int64_t n_rows, int64_t n_cols, int tr, int tc, bool upper,
+                                   int64_t row_begin, int64_t row_end, int group,
+                                   std::vector<size_t>* group_start);
+
+struct RandomIn {
+  int32_t mode;
+  int32_t n_clusters;
+  const float* table;
+};
+
+int check_random(const ppsk_set* rows, const ppsk_set* cols, const RandomIn& rnd);
+int tile_group();
+int default_variant();
+
+// Timing of the tile-kernel launches of one call: per-chunk events on the
+// launch stream, summed once the call has finished.
+struct Timing {
+  double ms = 0;
+  int64_t launches = 0;
+  int64_t items = 0;
+};
+
+hipEvent_t pool_event(DeviceCtx* c, size_t i);
+
+// Distance parameters shared by every entry point (rows = r1 of the Jaccard).
+void fill_common(ppsk::DistParams& p, const ppsk_set* rows, const ppsk_set* cols,
+                 const RandomIn& rnd, double tol);
+// Device copy of a Monte Carlo table (null for the other modes); caller frees.
+int upload_table(DeviceCtx* c, const ppsk_set* s, const RandomIn& rnd, float** dtab);
+
+}  // namespace capi
+}  // namespace ppsk

@@ -272,6 +272,10 @@ template <>
 struct ATuple<8> {
   typedef uint32_t T __attribute__((ext_vector_type(8)));
 };
+template <>
+struct ATuple<4> {
+  typedef uint32_t T __attribute__((ext_vector_type(4)));
+};
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 template <int RQ, int RR, int PB>
@@ -285,8 +289,11 @@ template <int RQ, int OFF>
 __device__ __forceinline__ void a_load(const CU32* aw, typename ATuple<RQ>::T& x) {
   if constexpr (RQ == 16) {
     asm volatile("s_load_dwordx16 %0, %1, %2" : "=&s"(x) : "s"(aw), "i"(OFF));
-  } else {
+  } else if constexpr (RQ == 8) {
     asm volatile("s_load_dwordx8 %0, %1, %2" : "=&s"(x) : "s"(aw), "i"(OFF));
+  } else {
+    static_assert(RQ == 4, "row operands come in SGPR tuples of 4, 8 or 16");
+    asm volatile("s_load_dwordx4 %0, %1, %2" : "=&s"(x) : "s"(aw), "i"(OFF));
   }
 }
 
@@ -405,9 +412,10 @@ __device__ __forceinline__ void match_stage_pipe(const CU32* arow, int w0, int n
 // per-k counts -> random-corrected Jaccard -> dense output.  Out of line so
 // that none of its state occupies registers during the matching loop.
 // ---------------------------------------------------------------------------
+// tile_counts: the tile's count block [n_k][tile rows][tile cols].
 template <int RQ, int RR>
-__device__ __noinline__ void tile_epilogue(CParams* pp, int64_t t, Tile tile, int wave,
-                                           int lane) {
+__device__ __noinline__ void tile_epilogue(CParams* pp, const uint16_t* tile_counts, Tile tile,
+                                           int wave, int lane) {
   CParams& p = *pp;
   using Cfg = TileCfg<RQ, RR>;
   constexpr int TC = Cfg::TC;
@@ -415,7 +423,7 @@ __device__ __noinline__ void tile_epilogue(CParams* pp, int64_t t, Tile tile, in
   const double bins = 64.0 * (double)p.s64;
   const int64_t gi0 = (int64_t)tile.bi * TROWS + wave * RQ;
   const int64_t col0 = (int64_t)tile.ct * TC;
-  const uint16_t* tbase = p.counts + (t * p.n_k * TROWS + wave * RQ) * (int64_t)TC + lane;
+  const uint16_t* tbase = tile_counts + (int64_t)(wave * RQ) * TC + lane;
   const int64_t kstride = (int64_t)TROWS * TC;
   for (int q = 0; q < RQ; ++q) {
     const int64_t gi = gi0 + q;
@@ -492,6 +500,7 @@ __device__ __noinline__ void tile_epilogue(CParams* pp, int64_t t, Tile tile, in
       if (first < 0 || last - first + 1 < 2) {
         atomicAdd(p.fail_count, 1ull);
         atomicMin(p.fail_first, (unsigned long long)(pos + p.out_row_offset));
+        if (p.fail_rows) atomicAdd(&p.fail_rows[gi], 1u);
       } else {
         const double n = (double)(last - first + 1);
         double sx = 0.0, sxx = 0.0;
@@ -516,51 +525,42 @@ __device__ __noinline__ void tile_epilogue(CParams* pp, int64_t t, Tile tile, in
 }
 
 // ---------------------------------------------------------------------------
-// The tile kernel: one (tile, k-mer length) item per workgroup.
+// Match counts of one tile for k-mer length index kk into cnt (zeroed here).
+// pp: the distance parameters, read with scalar loads (kernarg segment or a
+// job table entry).  sB0 / sB1: the two LDS stage buffers.
 // ---------------------------------------------------------------------------
 template <int RQ, int RR, int PB>
-__global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
+__device__ __forceinline__ void tile_match(CParams* pp, Tile tile, int kk, uint4* sB0,
+                                           uint4* sB1, int wave, int lane,
+                                           uint32_t (&cnt)[RQ][RR]) {
   using Cfg = TileCfg<RQ, RR>;
   constexpr int TC = Cfg::TC;
   constexpr int WS = Cfg::WS;
   constexpr int TROWS = Cfg::TROWS;
-  __shared__ uint4 sB0[Cfg::STAGE_U4];
-  __shared__ uint4 sB1[Cfg::STAGE_U4];
-  __shared__ int s_last;
-
-  const int64_t item = p.item_base + blockIdx.x;
-  const int64_t t = item / p.n_k;
-  const int kk = (int)(item - t * p.n_k);
-  Tile tile = p.tiles[t];
-  // wave-uniform values (kept in SGPRs)
-  tile.bi = __builtin_amdgcn_readfirstlane(tile.bi);
-  tile.ct = __builtin_amdgcn_readfirstlane(tile.ct);
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int s64 = p.s64;
+  const int s64 = pp->s64;
 
   // row operands: RQ consecutive samples of this wave, wave-uniform address,
   // read through the constant address space so they are scalar loads into
   // SGPRs (the stripes are never written while a distance kernel runs)
   const int64_t row0 = (int64_t)tile.bi * TROWS;
-  const CU32* arow = (CU32*)(p.row_stripes +
-                             ((int64_t)kk * p.nb_rows + row0 / BLK) * (int64_t)s64 * (HL * BLK) +
+  const CU32* arow = (CU32*)(pp->row_stripes +
+                             ((int64_t)kk * pp->nb_rows + row0 / BLK) * (int64_t)s64 * (HL * BLK) +
                              row0 % BLK + wave * RQ);
   // column slabs of this k-mer length.  The last stage may run past word
   // s64 - 1: the stripe buffer carries zero padding at its end, and those
   // words are never computed on.
   const int64_t col0 = (int64_t)tile.ct * TC;
+  const int nb_cols = pp->nb_cols;
   const uint32_t* __restrict__ col_kk =
-      p.col_stripes + (int64_t)kk * p.nb_cols * (int64_t)s64 * (HL * BLK);
+      pp->col_stripes + (int64_t)kk * nb_cols * (int64_t)s64 * (HL * BLK);
 
-  uint32_t cnt[RQ][RR];
 #pragma unroll
   for (int q = 0; q < RQ; ++q)
 #pragma unroll
     for (int r = 0; r < RR; ++r) cnt[q][r] = 0;
 
   const int n_stages = (s64 + WS - 1) / WS;
-  issue_stage<RQ, RR>(col_kk, col0, p.nb_cols, s64, 0, sB0, wave, lane);
+  issue_stage<RQ, RR>(col_kk, col0, nb_cols, s64, 0, sB0, wave, lane);
   // stages alternate between the two buffers; each pass of this loop runs an
   // even stage from sB0 and the following odd stage from sB1
   for (int st = 0; st < n_stages; st += 2) {
@@ -572,7 +572,7 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
         uint4* nxt = half ? sB0 : sB1;
         const int w0 = s * WS;
         if (s + 1 < n_stages) {
-          issue_stage<RQ, RR>(col_kk, col0, p.nb_cols, s64, w0 + WS, nxt, wave, lane);
+          issue_stage<RQ, RR>(col_kk, col0, nb_cols, s64, w0 + WS, nxt, wave, lane);
           wait_stage_but_one<RQ, RR>(wave);
         } else {
           wait_vmcnt<0>();
@@ -589,13 +589,50 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
       }
     }
   }
+}
 
-  // per-tile count scratch: [tile][kk][row][col]
-  uint16_t* cbase = p.counts + ((t * p.n_k + kk) * TROWS + wave * RQ) * (int64_t)TC + lane;
+// Counts of this wave / lane into a tile count block [n_k][rows][cols].
+template <int RQ, int RR>
+__device__ __forceinline__ void store_counts(uint16_t* tile_counts, int kk, int wave, int lane,
+                                             const uint32_t (&cnt)[RQ][RR]) {
+  using Cfg = TileCfg<RQ, RR>;
+  constexpr int TC = Cfg::TC;
+  uint16_t* cbase = tile_counts + ((int64_t)kk * Cfg::TROWS + wave * RQ) * TC + lane;
 #pragma unroll
   for (int q = 0; q < RQ; ++q)
 #pragma unroll
     for (int r = 0; r < RR; ++r) cbase[q * TC + 64 * r] = (uint16_t)cnt[q][r];
+}
+
+// ---------------------------------------------------------------------------
+// The tile kernel: one (tile, k-mer length) item per workgroup.
+// ---------------------------------------------------------------------------
+template <int RQ, int RR, int PB>
+__global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
+  using Cfg = TileCfg<RQ, RR>;
+  __shared__ uint4 sB0[Cfg::STAGE_U4];
+  __shared__ uint4 sB1[Cfg::STAGE_U4];
+  __shared__ int s_last;
+  // the parameters are the kernel's only argument: scalar loads straight
+  // from the kernarg segment (no copy)
+  CParams* pp = (CParams*)__builtin_amdgcn_kernarg_segment_ptr();
+
+  const int64_t item = p.item_base + blockIdx.x;
+  const int64_t t = item / p.n_k;
+  const int kk = (int)(item - t * p.n_k);
+  Tile tile = p.tiles[t];
+  // wave-uniform values (kept in SGPRs)
+  tile.bi = __builtin_amdgcn_readfirstlane(tile.bi);
+  tile.ct = __builtin_amdgcn_readfirstlane(tile.ct);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+
+  uint32_t cnt[RQ][RR];
+  tile_match<RQ, RR, PB>(pp, tile, kk, sB0, sB1, wave, lane, cnt);
+
+  // per-tile count scratch: [tile][kk][row][col]
+  uint16_t* tile_counts = p.counts + t * p.n_k * (int64_t)(Cfg::TROWS * Cfg::TC);
+  store_counts<RQ, RR>(tile_counts, kk, wave, lane, cnt);
 
   // completion: the last workgroup of this tile runs the epilogue
   __threadfence();
@@ -608,20 +645,115 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
   if (!s_last) return;
   __threadfence();
   if (threadIdx.x == 0) p.tile_done[t] = 0;  // ready for the next launch
-  // the parameters are the kernel's only argument: hand the epilogue the
-  // kernarg segment itself rather than a copy
-  tile_epilogue<RQ, RR>((CParams*)__builtin_amdgcn_kernarg_segment_ptr(), t, tile, wave, lane);
+  tile_epilogue<RQ, RR>(pp, tile_counts, tile, wave, lane);
 }
 
-bool variant_valid(int v) { return v >= 1 && v <= 5; }
+// ---------------------------------------------------------------------------
+// Persistent batch kernel: many query-vs-ref jobs (each with its own sets,
+// k-mer lengths, sketch size and output) in one launch.  Work items are
+// whole tiles (every k-mer length of the tile, then its epilogue) taken from
+// NQ queues; workgroup b starts on queue b % NQ (the XCD it is dispatched
+// to) and, once that queue is drained, moves on to the next ones in turn,
+// never returning to a drained queue; it exits after the last one.  Counts
+// go to the workgroup's own scratch block and are read back by the same
+// lanes in the epilogue, so no cross-workgroup synchronisation is needed.
+// ---------------------------------------------------------------------------
+// One (tile, k-mer length) of a batch item: counts into the workgroup's
+// scratch block.  Out of line so that the matching loop is register-allocated
+// on its own, exactly as in dist_tiles, instead of around the persistent
+// loop's state (which would push the counts out to scratch memory on every
+// word).  Arguments arrive in VGPRs and are made wave-uniform again here.
+template <int RQ, int RR, int PB>
+__device__ __noinline__ void batch_tile_k(CParams* pp_in, int bi_in, int ct_in, int kk_in,
+                                          uint16_t* scratch_in) {
+  using Cfg = TileCfg<RQ, RR>;
+  __shared__ uint4 sB0[Cfg::STAGE_U4];
+  __shared__ uint4 sB1[Cfg::STAGE_U4];
+  const uint64_t pv = (uint64_t)(uintptr_t)pp_in;
+  const uint32_t plo = __builtin_amdgcn_readfirstlane((uint32_t)pv);
+  const uint32_t phi = __builtin_amdgcn_readfirstlane((uint32_t)(pv >> 32));
+  CParams* pp = (CParams*)(uintptr_t)(((uint64_t)phi << 32) | plo);
+  const uint64_t sv = (uint64_t)(uintptr_t)scratch_in;
+  const uint32_t slo = __builtin_amdgcn_readfirstlane((uint32_t)sv);
+  const uint32_t shi = __builtin_amdgcn_readfirstlane((uint32_t)(sv >> 32));
+  uint16_t* scratch = (uint16_t*)(uintptr_t)(((uint64_t)shi << 32) | slo);
+  Tile tile;
+  tile.bi = __builtin_amdgcn_readfirstlane(bi_in);
+  tile.ct = __builtin_amdgcn_readfirstlane(ct_in);
+  const int kk = __builtin_amdgcn_readfirstlane(kk_in);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  uint32_t cnt[RQ][RR];
+  tile_match<RQ, RR, PB>(pp, tile, kk, sB0, sB1, wave, lane, cnt);
+  store_counts<RQ, RR>(scratch, kk, wave, lane, cnt);
+}
 
-int variant_tile_rows(int v) { return (v == 2 || v == 5) ? 64 : 128; }
+template <int RQ, int RR, int PB>
+__global__ __launch_bounds__(WAVES * 64, 4) void dist_batch(BatchArgs a) {
+  using Cfg = TileCfg<RQ, RR>;
+  __shared__ int64_t s_item;
+  __shared__ int s_tried;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int home = (int)(blockIdx.x % BATCH_QUEUES);
+  uint16_t* scratch =
+      a.scratch + (int64_t)blockIdx.x * a.max_k * (int64_t)(Cfg::TROWS * Cfg::TC);
+  int tried = 0;  // queues found drained so far (workgroup-uniform)
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int64_t got = -1;
+      while (tried < BATCH_QUEUES) {
+        const int q = (home + tried) % BATCH_QUEUES;
+        const int64_t size = a.queue_begin[q + 1] - a.queue_begin[q];
+        const unsigned idx = atomicAdd(&a.queue_next[q], 1u);
+        if ((int64_t)idx < size) {
+          got = a.queue_begin[q] + idx;
+          break;
+        }
+        ++tried;
+      }
+      s_item = got;
+      s_tried = tried;
+    }
+    __syncthreads();
+    const int64_t it = s_item;
+    tried = s_tried;
+    __syncthreads();  // s_item is rewritten only after every thread read it
+    if (it < 0) break;
+    const BatchItem bi = a.items[it];
+    const int job = __builtin_amdgcn_readfirstlane(bi.job);
+    Tile tile;
+    tile.bi = __builtin_amdgcn_readfirstlane(bi.bi);
+    tile.ct = __builtin_amdgcn_readfirstlane(bi.ct);
+    CParams* pp = (CParams*)(a.jobs + job);
+    const int n_k = pp->n_k;
+    for (int kk = 0; kk < n_k; ++kk) batch_tile_k<RQ, RR, PB>(pp, tile.bi, tile.ct, kk, scratch);
+    tile_epilogue<RQ, RR>(pp, scratch, tile, wave, lane);
+  }
+}
+
+__global__ void batch_reset_kernel(uint32_t* queue_next, unsigned long long* fail, int n_jobs) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < BATCH_QUEUES) queue_next[x] = 0u;
+  if (x < n_jobs) {
+    fail[2 * x] = 0ull;
+    fail[2 * x + 1] = ~0ull;
+  }
+}
+
+bool variant_valid(int v) { return v >= 1 && v <= N_VARIANTS; }
+
+int variant_tile_rows(int v) {
+  if (v == 6) return 32;
+  return (v == 2 || v == 5) ? 64 : 128;
+}
 
 int variant_tile_cols(int v) {
   switch (v) {
     case 1:
       return 64;
     case 2:
+    case 6:
       return 256;
     default:
       return 128;
@@ -644,10 +776,57 @@ hipError_t launch_dists(const DistParams& p, int variant, int64_t n_items, hipSt
     case 5:
       dist_tiles<8, 2, 2><<<g, WAVES * 64, 0, stream>>>(p);
       break;
+    case 6:
+      dist_tiles<4, 4, 2><<<g, WAVES * 64, 0, stream>>>(p);
+      break;
     default:
       dist_tiles<8, 4, 2><<<g, WAVES * 64, 0, stream>>>(p);
       break;
   }
+  return hipGetLastError();
+}
+
+// Batch kernels exist for the pipelined shapes only.
+template <typename F>
+static hipError_t with_batch_kernel(int variant, F f) {
+  switch (variant) {
+    case 5:
+      return f(dist_batch<8, 2, 2>);
+    case 6:
+      return f(dist_batch<4, 4, 2>);
+    case 2:
+      return f(dist_batch<8, 4, 2>);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t batch_grid(int variant, int* grid) {
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  return with_batch_kernel(variant, [&](auto kern) {
+    int per_cu = 0;
+    hipError_t r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WAVES * 64, 0);
+    if (r != hipSuccess) return r;
+    *grid = (per_cu > 0 ? per_cu : 1) * cus;
+    return hipSuccess;
+  });
+}
+
+hipError_t launch_batch(const BatchArgs& a, int variant, int grid, hipStream_t stream) {
+  if (grid <= 0) return hipErrorInvalidValue;
+  return with_batch_kernel(variant, [&](auto kern) {
+    kern<<<grid, WAVES * 64, 0, stream>>>(a);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_batch_reset(uint32_t* queue_next, unsigned long long* fail, int n_jobs,
+                              hipStream_t stream) {
+  const int n = n_jobs > BATCH_QUEUES ? n_jobs : BATCH_QUEUES;
+  batch_reset_kernel<<<(n + 255) / 256, 256, 0, stream>>>(queue_next, fail, n_jobs);
   return hipGetLastError();
 }
 

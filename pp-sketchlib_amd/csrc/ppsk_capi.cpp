@@ -16,49 +16,24 @@
 #include "../../include/ppsk.h"
 #include "ppsk_internal.h"
 #include "select_kernels.h"
+#include "capi_common.h"
 
-namespace {
+namespace ppsk {
+namespace capi {
 
 thread_local std::string g_last_error;
 thread_local double g_last_ms = 0.0;
 thread_local int64_t g_last_launches = 0;
 thread_local int64_t g_last_items = 0;
+std::mutex g_ctx_mu;
+std::vector<DeviceCtx*> g_ctx;
 
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
 
-#define HIP_TRY(expr)                                                              \
-  do {                                                                             \
-    hipError_t _e = (expr);                                                        \
-    if (_e != hipSuccess) {                                                        \
-      return fail(_e == hipErrorOutOfMemory ? PPSK_ERR_MEMORY : PPSK_ERR_HIP,      \
-                  std::string(#expr) + ": " + hipGetErrorString(_e));              \
-    }                                                                              \
-  } while (0)
 
-// Per-device state: a launch stream, a copy stream, cached workspaces grown on
-// demand.
-struct DeviceCtx {
-  int device = -1;
-  hipStream_t stream = nullptr;
-  void* counts = nullptr;
-  size_t counts_bytes = 0;
-  uint32_t* tile_done = nullptr;
-  size_t tile_done_n = 0;
-  ppsk::Tile* tiles = nullptr;
-  size_t tiles_n = 0;
-  unsigned long long* fail = nullptr;  // [0] count, [1] first row
-  void* out = nullptr;
-  size_t out_bytes = 0;
-  hipStream_t copy_stream = nullptr;  // device -> host copies of finished rows
-  std::vector<hipEvent_t> events;     // per-chunk timing / copy dependencies
-  std::mutex mu;
-};
-
-std::mutex g_ctx_mu;
-std::vector<DeviceCtx*> g_ctx;
 
 int get_ctx(int device, DeviceCtx** out) {
   int n = 0;
@@ -104,25 +79,12 @@ int grow_tile_done(DeviceCtx* c, size_t n) {
   return PPSK_OK;
 }
 
-}  // namespace
+}  // namespace capi
+}  // namespace ppsk
 
-struct ppsk_set {
-  int device;
-  int64_t n;
-  int32_t n_k;
-  std::vector<int32_t> klist;
-  int32_t s64;
-  int32_t bbits;
-  int32_t nb;
-  uint32_t* stripes = nullptr;  // [n_k][nb][s64][28][128]
-  double* jv = nullptr;         // [n][n_k]
-  uint16_t* clusters = nullptr; // [n]
-  bool has_lengths = false;
-  int32_t use_rc = 1;
-  int32_t max_cluster = -1;     // largest cluster id uploaded
-};
 
-namespace {
+namespace ppsk {
+namespace capi {
 
 int check_set(const ppsk_set* s, const char* what) {
   if (!s) return fail(PPSK_ERR_ARG, std::string(what) + " is NULL");
@@ -172,11 +134,6 @@ std::vector<ppsk::Tile> make_tiles(int64_t n_rows, int64_t n_cols, int tr, int t
   return t;
 }
 
-struct RandomIn {
-  int32_t mode;
-  int32_t n_clusters;
-  const float* table;
-};
 
 int check_random(const ppsk_set* rows, const ppsk_set* cols, const RandomIn& rnd) {
   if (rnd.mode == PPSK_RANDOM_EQUAL_BASE) {
@@ -221,13 +178,6 @@ int default_variant() {
   return v;
 }
 
-// Timing of the tile-kernel launches of one call: per-chunk events on the
-// launch stream, summed once the call has finished.
-struct Timing {
-  double ms = 0;
-  int64_t launches = 0;
-  int64_t items = 0;
-};
 
 hipEvent_t pool_event(DeviceCtx* c, size_t i) {
   while (c->events.size() <= i) {
@@ -495,7 +445,10 @@ void set_timing(const Timing& tm) {
   g_last_items = tm.items;
 }
 
-}  // namespace
+}  // namespace capi
+}  // namespace ppsk
+
+using namespace ppsk::capi;
 
 extern "C" {
 
@@ -564,6 +517,7 @@ int ppsk_set_create(int device_id, int64_t n_samples, int32_t n_k, const int32_t
 
 int ppsk_set_upload(ppsk_set* s, int64_t first, int64_t count, const uint64_t* raw) {
   if (int rc = check_set(s, "set")) return rc;
+  if (s->view) return fail(PPSK_ERR_STATE, "a prefix view is read-only");
   if (first < 0 || count < 0 || first + count > s->n) {
     return fail(PPSK_ERR_ARG, "upload range outside the set");
   }
@@ -593,6 +547,7 @@ int ppsk_set_upload(ppsk_set* s, int64_t first, int64_t count, const uint64_t* r
 
 int ppsk_set_meta(ppsk_set* s, const uint64_t* lengths, int32_t use_rc, const uint16_t* clusters) {
   if (int rc = check_set(s, "set")) return rc;
+  if (s->view) return fail(PPSK_ERR_STATE, "a prefix view is read-only");
   DeviceCtx* c = nullptr;
   if (int rc = get_ctx(s->device, &c)) return rc;
   if (lengths) {
@@ -629,8 +584,28 @@ int ppsk_set_meta(ppsk_set* s, const uint64_t* lengths, int32_t use_rc, const ui
   return PPSK_OK;
 }
 
+int ppsk_set_prefix(ppsk_set* parent, int64_t count, ppsk_set** out) {
+  if (!out) return fail(PPSK_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (int rc = check_set(parent, "parent")) return rc;
+  if (count < 1 || count > parent->n) {
+    return fail(PPSK_ERR_ARG, "prefix length must be in [1, set size]");
+  }
+  // same buffers and stripe-block stride; only the sample count shrinks, so
+  // every kernel index stays inside the parent's allocation
+  auto* v = new ppsk_set(*parent);
+  v->n = count;
+  v->view = true;
+  *out = v;
+  return PPSK_OK;
+}
+
 int ppsk_set_free(ppsk_set* s) {
   if (!s) return PPSK_OK;
+  if (s->view) {
+    delete s;
+    return PPSK_OK;
+  }
   hipSetDevice(s->device);
   if (s->stripes) hipFree(s->stripes);
   if (s->jv) hipFree(s->jv);

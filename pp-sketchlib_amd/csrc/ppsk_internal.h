@@ -68,6 +68,26 @@ struct DistParams {
   const uint16_t* cl_cols;
   unsigned long long* fail_count;
   unsigned long long* fail_first;  // smallest failing dense row
+  uint32_t* fail_rows;             // optional [n_rows] failures per tile row sample
+};
+
+// Persistent batch kernel (dist_batch): work queues and items.
+constexpr int BATCH_QUEUES = 8;  // one per XCD of the MI355X
+
+struct BatchItem {  // one tile of one job: every k-mer length, then the epilogue
+  int32_t job;
+  int32_t bi;
+  int32_t ct;
+  int32_t pad;
+};
+
+struct BatchArgs {
+  const DistParams* jobs;   // [n_jobs] per-job parameters (device memory)
+  const BatchItem* items;   // queue q holds items [queue_begin[q], queue_begin[q + 1])
+  int64_t queue_begin[BATCH_QUEUES + 1];
+  uint32_t* queue_next;     // [BATCH_QUEUES] next item of each queue (zeroed before launch)
+  uint16_t* scratch;        // [grid][max_k][tile rows][tile cols] per-workgroup counts
+  int32_t max_k;
 };
 
 // Launchers (dist_kernels.hip).  All work goes on `stream`.
@@ -80,12 +100,22 @@ hipError_t launch_stripe(const uint64_t* raw, int64_t count, int64_t first,
 // compiler, 1 or 2 = explicit double-buffered scalar loads of PB planes).
 //   1: RQ16 RR1 PB0 (128 x 64)     3: RQ16 RR2 PB0 (128 x 128)
 //   2: RQ8  RR4 PB2 (64 x 256)     4: RQ16 RR2 PB1 (128 x 128)
-//   5: RQ8  RR2 PB2 (64 x 128)
+//   5: RQ8  RR2 PB2 (64 x 128)     6: RQ4  RR4 PB2 (32 x 256)
 constexpr int DEFAULT_VARIANT = 2;
+constexpr int N_VARIANTS = 6;
 bool variant_valid(int variant);
 int variant_tile_rows(int variant);
 int variant_tile_cols(int variant);
 hipError_t launch_dists(const DistParams& p, int variant, int64_t n_items,
                         hipStream_t stream);
+
+// Persistent batch launch: `grid` workgroups (batch_grid: every workgroup
+// resident at once).  batch_reset zeroes the queue counters and sets the
+// per-job fail counters ([n_jobs][2]: count, first row) to (0, max) with a
+// kernel, so that both launches can be captured in a graph.
+hipError_t batch_grid(int variant, int* grid);
+hipError_t launch_batch(const BatchArgs& a, int variant, int grid, hipStream_t stream);
+hipError_t launch_batch_reset(uint32_t* queue_next, unsigned long long* fail, int n_jobs,
+                              hipStream_t stream);
 
 }  // namespace ppsk

@@ -20,6 +20,25 @@ hipError_t launch_knn_select_triangle(const float* tri, int64_t n, int dist_col,
                                       int64_t* i_vec, int64_t* j_vec, float* dists,
                                       hipStream_t stream);
 
+// Query rows against a separate reference set: band [n_rows][n] float2,
+// nothing excluded; i_vec holds the band row.
+hipError_t launch_knn_select_rect(const float* band, int64_t n_rows, int64_t n, int dist_col,
+                                  int kNN, int64_t* i_vec, int64_t* j_vec, float* dists,
+                                  hipStream_t stream);
+
+// Threshold in two passes over `rows` rows: per-row counts of d < cutoff,
+// then (after an exclusive scan of the counts into offsets) the entries in
+// row-major order.  rect: rows of n entries of a query-major band (j = column);
+// otherwise the condensed self triangle of n samples (rows = n - 1, j > i).
+// Outputs may be device or page-locked host memory.
+hipError_t launch_threshold_count(const float2* dd, int64_t rows, int64_t n, bool rect,
+                                  int dist_col, float cutoff, int64_t* counts,
+                                  hipStream_t stream);
+hipError_t launch_threshold_write(const float2* dd, int64_t rows, int64_t n, bool rect,
+                                  int dist_col, float cutoff, const int64_t* offsets,
+                                  int64_t* i_vec, int64_t* j_vec, float* dists,
+                                  hipStream_t stream);
+
 // dd: condensed float2 (core, acc) of the self upper triangle of n samples.
 // Allocates the device outputs (*d_i, *d_j, *d_d; caller frees with hipFree).
 hipError_t threshold_compact(const float2* dd, int64_t n, int dist_col, float cutoff,

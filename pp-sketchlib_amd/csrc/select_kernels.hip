@@ -91,9 +91,12 @@ struct TriangleRows {  // condensed upper triangle of n samples (symmetric dista
   }
 };
 
+// excl: column gi is the row's own sample (+inf, self kNN); otherwise the
+// rows are queries against a separate reference set and nothing is excluded.
 template <typename Rows>
 __global__ __launch_bounds__(64) void knn_select_kernel(Rows rows, int64_t r0, int64_t n,
-                                                        int kNN, int64_t* __restrict__ iv,
+                                                        int kNN, bool excl,
+                                                        int64_t* __restrict__ iv,
                                                         int64_t* __restrict__ jv,
                                                         float* __restrict__ dv) {
   extern __shared__ HeapEnt heap[];
@@ -105,7 +108,7 @@ __global__ __launch_bounds__(64) void knn_select_kernel(Rows rows, int64_t r0, i
   for (int64_t c0 = 0; c0 < n; c0 += 64) {
     const int64_t j = c0 + lane;
     float d = INFINITY;
-    if (j < n && j != gi) d = rows(gi, j);
+    if (j < n && !(excl && j == gi)) d = rows(gi, j);
     const bool cand = (j < n) && (len < kNN || d < top);
     unsigned long long mask = __ballot(cand);
     while (mask) {
@@ -145,7 +148,17 @@ hipError_t launch_knn_select(const float* band, int64_t r0, int64_t n_rows, int6
   if (n_rows <= 0) return hipSuccess;
   // push writes slot kNN before the pop trims the heap back to kNN entries
   knn_select_kernel<BandRows><<<(unsigned)n_rows, 64, (kNN + 1) * sizeof(HeapEnt), stream>>>(
-      BandRows{band, r0, n, dist_col}, r0, n, kNN, i_vec, j_vec, dists);
+      BandRows{band, r0, n, dist_col}, r0, n, kNN, true, i_vec, j_vec, dists);
+  return hipGetLastError();
+}
+
+hipError_t launch_knn_select_rect(const float* band, int64_t n_rows, int64_t n, int dist_col,
+                                  int kNN, int64_t* i_vec, int64_t* j_vec, float* dists,
+                                  hipStream_t stream) {
+  if (kNN < 1 || kNN > KNN_MAX || kNN > n) return hipErrorInvalidValue;
+  if (n_rows <= 0) return hipSuccess;
+  knn_select_kernel<BandRows><<<(unsigned)n_rows, 64, (kNN + 1) * sizeof(HeapEnt), stream>>>(
+      BandRows{band, 0, n, dist_col}, 0, n, kNN, false, i_vec, j_vec, dists);
   return hipGetLastError();
 }
 
@@ -155,26 +168,37 @@ hipError_t launch_knn_select_triangle(const float* tri, int64_t n, int dist_col,
   if (kNN < 1 || kNN > KNN_MAX || kNN > n) return hipErrorInvalidValue;
   if (n <= 0) return hipSuccess;
   knn_select_kernel<TriangleRows><<<(unsigned)n, 64, (kNN + 1) * sizeof(HeapEnt), stream>>>(
-      TriangleRows{tri, n, dist_col}, 0, n, kNN, i_vec, j_vec, dists);
+      TriangleRows{tri, n, dist_col}, 0, n, kNN, true, i_vec, j_vec, dists);
   return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Threshold compaction over the condensed self triangle.
-// Row i covers condensed positions [n*i - i(i+1)/2, + n-1-i), j = i+1+t.
+// Threshold compaction, row by row.  Triangle: the condensed self triangle,
+// row i covering positions [n*i - i(i+1)/2, + n-1-i) with j = i+1+t.
+// Rectangle: rows of n_cols entries (query-major band), j = t.
 // ---------------------------------------------------------------------------
 constexpr int TB = 256;
 
 __device__ __forceinline__ float pick(const float2 v, int col) { return col ? v.y : v.x; }
 
+struct RowSpan {
+  int64_t start, len, j0;
+};
+
+__device__ __forceinline__ RowSpan row_span(bool rect, int64_t i, int64_t n) {
+  if (rect) return RowSpan{i * n, n, 0};
+  return RowSpan{n * i - ((i * (i + 1)) >> 1), n - 1 - i, i + 1};
+}
+
 __global__ __launch_bounds__(TB) void thr_count_kernel(const float2* __restrict__ dd, int64_t n,
-                                                       int col, float cutoff,
+                                                       bool rect, int col, float cutoff,
                                                        int64_t* __restrict__ counts) {
   const int64_t i = blockIdx.x;
-  const int64_t start = n * i - ((i * (i + 1)) >> 1);
-  const int64_t len = n - 1 - i;
+  const RowSpan sp = row_span(rect, i, n);
   int64_t c = 0;
-  for (int64_t t = threadIdx.x; t < len; t += TB) c += pick(dd[start + t], col) < cutoff ? 1 : 0;
+  for (int64_t t = threadIdx.x; t < sp.len; t += TB) {
+    c += pick(dd[sp.start + t], col) < cutoff ? 1 : 0;
+  }
   __shared__ int64_t red[TB / 64];
   // wave reduction then across waves
   for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
@@ -188,26 +212,25 @@ __global__ __launch_bounds__(TB) void thr_count_kernel(const float2* __restrict_
 }
 
 __global__ __launch_bounds__(TB) void thr_write_kernel(const float2* __restrict__ dd, int64_t n,
-                                                       int col, float cutoff,
+                                                       bool rect, int col, float cutoff,
                                                        const int64_t* __restrict__ offsets,
                                                        int64_t* __restrict__ iv,
                                                        int64_t* __restrict__ jv,
                                                        float* __restrict__ dv) {
   const int64_t i = blockIdx.x;
-  const int64_t start = n * i - ((i * (i + 1)) >> 1);
-  const int64_t len = n - 1 - i;
+  const RowSpan sp = row_span(rect, i, n);
   __shared__ int wave_tot[TB / 64];
   __shared__ int64_t base_sh;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   if (threadIdx.x == 0) base_sh = offsets[i];
   __syncthreads();
-  for (int64_t c0 = 0; c0 < len; c0 += TB) {
+  for (int64_t c0 = 0; c0 < sp.len; c0 += TB) {
     const int64_t t = c0 + threadIdx.x;
     float d = 0.0f;
     bool keep = false;
-    if (t < len) {
-      d = pick(dd[start + t], col);
+    if (t < sp.len) {
+      d = pick(dd[sp.start + t], col);
       keep = d < cutoff;
     }
     const unsigned long long m = __ballot(keep);
@@ -224,13 +247,31 @@ __global__ __launch_bounds__(TB) void thr_write_kernel(const float2* __restrict_
     if (keep) {
       const int64_t o = base + wave_off + before;
       iv[o] = i;
-      jv[o] = i + 1 + t;
+      jv[o] = sp.j0 + t;
       dv[o] = d;
     }
     __syncthreads();
     if (threadIdx.x == 0) base_sh = base + chunk_tot;
     __syncthreads();
   }
+}
+
+hipError_t launch_threshold_count(const float2* dd, int64_t rows, int64_t n, bool rect,
+                                  int dist_col, float cutoff, int64_t* counts,
+                                  hipStream_t stream) {
+  if (rows <= 0) return hipSuccess;
+  thr_count_kernel<<<(unsigned)rows, TB, 0, stream>>>(dd, n, rect, dist_col, cutoff, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_threshold_write(const float2* dd, int64_t rows, int64_t n, bool rect,
+                                  int dist_col, float cutoff, const int64_t* offsets,
+                                  int64_t* i_vec, int64_t* j_vec, float* dists,
+                                  hipStream_t stream) {
+  if (rows <= 0) return hipSuccess;
+  thr_write_kernel<<<(unsigned)rows, TB, 0, stream>>>(dd, n, rect, dist_col, cutoff, offsets,
+                                                       i_vec, j_vec, dists);
+  return hipGetLastError();
 }
 
 hipError_t threshold_compact(const float2* dd, int64_t n, int dist_col, float cutoff,
@@ -245,8 +286,7 @@ hipError_t threshold_compact(const float2* dd, int64_t n, int dist_col, float cu
   int64_t* counts = nullptr;
   hipError_t e = hipMalloc(&counts, rows * sizeof(int64_t));
   if (e != hipSuccess) return e;
-  thr_count_kernel<<<(unsigned)rows, TB, 0, stream>>>(dd, n, dist_col, cutoff, counts);
-  e = hipGetLastError();
+  e = launch_threshold_count(dd, rows, n, false, dist_col, cutoff, counts, stream);
   std::vector<int64_t> h(rows);
   if (e == hipSuccess) e = hipMemcpyAsync(h.data(), counts, rows * sizeof(int64_t),
                                           hipMemcpyDeviceToHost, stream);
@@ -268,9 +308,8 @@ hipError_t threshold_compact(const float2* dd, int64_t n, int dist_col, float cu
   if (e == hipSuccess) e = hipMalloc(d_j, m * sizeof(int64_t));
   if (e == hipSuccess) e = hipMalloc(d_d, m * sizeof(float));
   if (e == hipSuccess && run > 0) {
-    thr_write_kernel<<<(unsigned)rows, TB, 0, stream>>>(dd, n, dist_col, cutoff, counts, *d_i,
-                                                         *d_j, *d_d);
-    e = hipGetLastError();
+    e = launch_threshold_write(dd, rows, n, false, dist_col, cutoff, counts, *d_i, *d_j, *d_d,
+                               stream);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   hipFree(counts);

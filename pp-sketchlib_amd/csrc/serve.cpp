@@ -1,0 +1,624 @@
+// Serving (include/ppsk.h, ppsk_server_*): a reference set stays resident in
+// HBM and query batches arrive at any time.  Continuous batching:
+//
+//  * every request waits in one FIFO queue;
+//  * n_slots worker threads, each with its own stream, repeatedly take as
+//    many queued requests as fit one step (at most max_rows query samples
+//    and the step's device memory, carved from one arena allocated at
+//    start-up), so requests that arrive while other steps are on the GPU are
+//    admitted as soon as a slot and memory free up, and steps of different
+//    slots overlap on the device;
+//  * the queries of all requests in a step share the row tiles of one
+//    query-vs-reference distance pass (the dense band never leaves the
+//    device), followed by kNN selection and threshold compaction straight
+//    into page-locked host buffers;
+//  * each request's rows are a contiguous slice of the step, so its results
+//    are the slice of the step's results, re-indexed to the request.
+//
+// Results do not depend on the step a request lands in: every output value
+// is a function of its own (query, reference) pair only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "capi_common.h"
+#include "select_kernels.h"
+
+using namespace ppsk::capi;
+
+namespace {
+
+constexpr size_t ARENA_ALIGN = 256;
+
+size_t align_up(size_t x) { return (x + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN; }
+
+// First-fit allocator over one device allocation (host-side bookkeeping
+// only, so steps never call hipMalloc).
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> length, coalesced
+  size_t used = 0, peak = 0;
+
+  void init(char* b, size_t n) {
+    base = b;
+    size = n;
+    free_.clear();
+    free_[0] = n;
+  }
+  bool fits(size_t n) const {
+    for (auto& f : free_) if (f.second >= n) return true;
+    return false;
+  }
+  bool alloc(size_t n, size_t* off) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < n) continue;
+      *off = it->first;
+      const size_t rest = it->second - n;
+      const size_t at = it->first + n;
+      free_.erase(it);
+      if (rest) free_[at] = rest;
+      used += n;
+      peak = std::max(peak, used);
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t n) {
+    used -= n;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct ppsk_request {
+  int64_t ticket = 0;
+  int64_t n_q = 0;
+  std::vector<uint64_t> raw;  // [n_q][n_k][s64 * 14]
+  std::vector<double> jv;     // [n_q][n_k] equal-base terms
+  ppsk_result res;            // public view of the vectors below
+  std::vector<int64_t> ki, kj, ti, tj;
+  std::vector<float> kd, td;
+  bool done = false;
+  int status = PPSK_OK;
+  std::string error;
+};
+
+struct ppsk_server {
+  ppsk_set* ref = nullptr;
+  int device = 0;
+  int32_t random_mode = 0;
+  double tolerance = 0;
+  int32_t kNN = 0, knn_col = 0, thr_col = 0;
+  float cutoff = 0;
+  int32_t max_rows = 0;
+  int variant = 2;
+  size_t per_sample = 0;
+  size_t scratch_budget = 0;
+  Arena arena;
+  void* arena_mem = nullptr;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<ppsk_request*> pending;
+  std::map<int64_t, ppsk_request*> tickets;  // submitted, not yet collected
+  int64_t next_ticket = 1;
+  bool stop = false;
+  std::vector<std::thread> workers;
+  std::chrono::steady_clock::time_point t0;
+  // statistics
+  int64_t steps = 0, rows_done = 0, requests_done = 0;
+  int32_t inflight = 0, peak_inflight = 0;
+  double gpu_ms = 0;  // summed distance-kernel time of all steps
+};
+
+namespace {
+
+struct StepLayout {
+  size_t staging, stripes, jv, band, tiles, scratch, tile_done, fail, fail_rows, total;
+  int64_t nbq, n_tiles, chunk_tiles;
+};
+
+StepLayout layout(const ppsk_server* s, int64_t rows) {
+  const ppsk_set* r = s->ref;
+  StepLayout L{};
+  const int tr = ppsk::variant_tile_rows(s->variant);
+  const int tc = ppsk::variant_tile_cols(s->variant);
+  L.nbq = (rows + ppsk::BLK - 1) / ppsk::BLK;
+  L.n_tiles = ((rows + tr - 1) / tr) * ((r->n + tc - 1) / tc);
+  const size_t per_tile = (size_t)r->n_k * tr * tc * sizeof(uint16_t);
+  L.chunk_tiles = std::max<int64_t>(1, std::min<int64_t>(L.n_tiles, s->scratch_budget / per_tile));
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += align_up(std::max<size_t>(bytes, 1));
+    return at;
+  };
+  L.staging = take((size_t)rows * s->per_sample);
+  L.stripes = take((size_t)r->n_k * L.nbq * r->s64 * ppsk::HL * ppsk::BLK * sizeof(uint32_t));
+  L.jv = take((size_t)rows * r->n_k * sizeof(double));
+  L.band = take((size_t)rows * r->n * 2 * sizeof(float));
+  L.tiles = take((size_t)L.n_tiles * sizeof(ppsk::Tile));
+  L.scratch = take((size_t)L.chunk_tiles * per_tile);
+  L.tile_done = take((size_t)L.chunk_tiles * sizeof(uint32_t));
+  L.fail = take(2 * sizeof(unsigned long long));
+  L.fail_rows = take((size_t)rows * sizeof(uint32_t));
+  L.total = off;
+  return L;
+}
+
+// Page-locked host buffer that grows on demand.
+struct Pinned {
+  void* p = nullptr;
+  size_t n = 0;
+  bool reserve(size_t bytes) {
+    if (bytes <= n && p) return true;
+    if (p) hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    if (hipHostMalloc(&p, std::max<size_t>(bytes, 64), hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return false;
+    }
+    n = std::max<size_t>(bytes, 64);
+    return true;
+  }
+  ~Pinned() {
+    if (p) hipHostFree(p);
+  }
+};
+
+struct Worker {
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  Pinned knn_i, knn_j, knn_d, counts, thr_i, thr_j, thr_d, fail_rows;
+};
+
+#define STEP_TRY(expr)                                                 \
+  do {                                                                 \
+    hipError_t _e = (expr);                                            \
+    if (_e != hipSuccess) {                                            \
+      err = std::string(#expr) + ": " + hipGetErrorString(_e);         \
+      return PPSK_ERR_HIP;                                             \
+    }                                                                  \
+  } while (0)
+
+// One step over `batch` (rows contiguous in submission order).
+int run_step(ppsk_server* s, Worker& w, const std::vector<ppsk_request*>& batch, int64_t rows,
+             char* mem, const StepLayout& L, std::string& err) {
+  const ppsk_set* ref = s->ref;
+  const int n_k = ref->n_k;
+  uint64_t* staging = (uint64_t*)(mem + L.staging);
+  uint32_t* stripes = (uint32_t*)(mem + L.stripes);
+  double* jv = (double*)(mem + L.jv);
+  float* band = (float*)(mem + L.band);
+  ppsk::Tile* d_tiles = (ppsk::Tile*)(mem + L.tiles);
+  uint16_t* scratch = (uint16_t*)(mem + L.scratch);
+  uint32_t* tile_done = (uint32_t*)(mem + L.tile_done);
+  unsigned long long* fail = (unsigned long long*)(mem + L.fail);
+  uint32_t* fail_rows = (uint32_t*)(mem + L.fail_rows);
+
+  // queries -> stripes (+ equal-base terms)
+  int64_t o = 0;
+  for (ppsk_request* r : batch) {
+    STEP_TRY(hipMemcpyAsync(staging + (size_t)o * (s->per_sample / sizeof(uint64_t)), r->raw.data(),
+                            (size_t)r->n_q * s->per_sample, hipMemcpyHostToDevice, w.st));
+    if (!r->jv.empty()) {
+      STEP_TRY(hipMemcpyAsync(jv + (size_t)o * n_k, r->jv.data(), r->jv.size() * sizeof(double),
+                              hipMemcpyHostToDevice, w.st));
+    }
+    o += r->n_q;
+  }
+  STEP_TRY(ppsk::launch_stripe(staging, rows, 0, n_k, ref->s64, (int32_t)L.nbq, stripes, w.st));
+  STEP_TRY(hipMemsetAsync(tile_done, 0, (size_t)L.chunk_tiles * sizeof(uint32_t), w.st));
+  STEP_TRY(hipMemsetAsync(fail, 0, sizeof(unsigned long long), w.st));
+  STEP_TRY(hipMemsetAsync(fail + 1, 0xFF, sizeof(unsigned long long), w.st));
+  STEP_TRY(hipMemsetAsync(fail_rows, 0, (size_t)rows * sizeof(uint32_t), w.st));
+
+  // distances: query rows x every reference, dense band in device memory
+  ppsk::DistParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.row_stripes = stripes;
+  p.col_stripes = ref->stripes;
+  p.n_rows = rows;
+  p.n_cols = ref->n;
+  p.nb_rows = (int32_t)L.nbq;
+  p.nb_cols = ref->nb;
+  p.n_k = n_k;
+  p.s64 = ref->s64;
+  for (int k = 0; k < n_k; ++k) p.kvals[k] = (float)ref->klist[k];
+  p.tolerance = s->tolerance;
+  p.random_mode = s->random_mode;
+  p.jv_rows = s->random_mode == PPSK_RANDOM_EQUAL_BASE ? jv : nullptr;
+  p.jv_cols = ref->jv;
+  p.mc_ref_is_col = 1;
+  p.out_kind = PPSK_OUT_CORE_ACC;
+  p.out = band;
+  p.counts = scratch;
+  p.tile_done = tile_done;
+  p.fail_count = fail;
+  p.fail_first = fail + 1;
+  p.fail_rows = fail_rows;
+  const int tr = ppsk::variant_tile_rows(s->variant);
+  const int tc = ppsk::variant_tile_cols(s->variant);
+  std::vector<size_t> gs;
+  const auto tiles = make_tiles(rows, ref->n, tr, tc, false, 0, rows, tile_group(), &gs);
+  STEP_TRY(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(ppsk::Tile),
+                          hipMemcpyHostToDevice, w.st));
+  STEP_TRY(hipEventRecord(w.e0, w.st));
+  for (size_t a = 0; a < tiles.size(); a += (size_t)L.chunk_tiles) {
+    const size_t b = std::min(tiles.size(), a + (size_t)L.chunk_tiles);
+    p.tiles = d_tiles + a;
+    p.n_tiles = (int64_t)(b - a);
+    p.item_base = 0;
+    STEP_TRY(ppsk::launch_dists(p, s->variant, (int64_t)(b - a) * n_k, w.st));
+  }
+  STEP_TRY(hipEventRecord(w.e1, w.st));
+
+  // kNN and threshold counts straight into page-locked host memory
+  const size_t nk = (size_t)rows * s->kNN;
+  if (s->kNN > 0) {
+    if (!w.knn_i.reserve(nk * 8) || !w.knn_j.reserve(nk * 8) || !w.knn_d.reserve(nk * 4)) {
+      err = "page-locked kNN buffers";
+      return PPSK_ERR_MEMORY;
+    }
+    STEP_TRY(ppsk::launch_knn_select_rect(band, rows, ref->n, s->knn_col, s->kNN,
+                                          (int64_t*)w.knn_i.p, (int64_t*)w.knn_j.p,
+                                          (float*)w.knn_d.p, w.st));
+  }
+  if (!w.counts.reserve((size_t)(rows + 1) * 8) || !w.fail_rows.reserve((size_t)rows * 4)) {
+    err = "page-locked count buffers";
+    return PPSK_ERR_MEMORY;
+  }
+  int64_t* cnt = (int64_t*)w.counts.p;
+  if (s->cutoff > 0) {
+    STEP_TRY(ppsk::launch_threshold_count((const float2*)band, rows, ref->n, true, s->thr_col,
+                                          s->cutoff, cnt, w.st));
+  }
+  STEP_TRY(hipMemcpyAsync(w.fail_rows.p, fail_rows, (size_t)rows * 4, hipMemcpyDeviceToHost, w.st));
+  STEP_TRY(hipStreamSynchronize(w.st));
+  float ms = 0.f;
+  STEP_TRY(hipEventElapsedTime(&ms, w.e0, w.e1));
+  int64_t total = 0;
+  if (s->cutoff > 0) {
+    for (int64_t x = 0; x < rows; ++x) {
+      const int64_t c = cnt[x];
+      cnt[x] = total;
+      total += c;
+    }
+    cnt[rows] = total;
+    if (total > 0) {
+      if (!w.thr_i.reserve((size_t)total * 8) || !w.thr_j.reserve((size_t)total * 8) ||
+          !w.thr_d.reserve((size_t)total * 4)) {
+        err = "page-locked threshold buffers";
+        return PPSK_ERR_MEMORY;
+      }
+      STEP_TRY(ppsk::launch_threshold_write((const float2*)band, rows, ref->n, true, s->thr_col,
+                                            s->cutoff, cnt, (int64_t*)w.thr_i.p,
+                                            (int64_t*)w.thr_j.p, (float*)w.thr_d.p, w.st));
+      STEP_TRY(hipStreamSynchronize(w.st));
+    }
+  }
+
+  // each request's slice, re-indexed to its own query numbering
+  const uint32_t* fr = (const uint32_t*)w.fail_rows.p;
+  o = 0;
+  for (ppsk_request* r : batch) {
+    const int64_t q0 = o, q1 = o + r->n_q;
+    if (s->kNN > 0) {
+      const int64_t* si = (const int64_t*)w.knn_i.p;
+      const int64_t* sj = (const int64_t*)w.knn_j.p;
+      const float* sd = (const float*)w.knn_d.p;
+      const size_t a = (size_t)q0 * s->kNN, b = (size_t)q1 * s->kNN;
+      r->ki.resize(b - a);
+      for (size_t x = a; x < b; ++x) r->ki[x - a] = si[x] - q0;
+      r->kj.assign(sj + a, sj + b);
+      r->kd.assign(sd + a, sd + b);
+    }
+    if (s->cutoff > 0 && total > 0) {
+      const int64_t a = cnt[q0], b = cnt[q1];
+      const int64_t* si = (const int64_t*)w.thr_i.p;
+      r->ti.resize(b - a);
+      for (int64_t x = a; x < b; ++x) r->ti[x - a] = si[x] - q0;
+      r->tj.assign((const int64_t*)w.thr_j.p + a, (const int64_t*)w.thr_j.p + b);
+      r->td.assign((const float*)w.thr_d.p + a, (const float*)w.thr_d.p + b);
+    }
+    uint64_t nf = 0;
+    for (int64_t x = q0; x < q1; ++x) nf += fr[x];
+    r->res.n_fail = nf;
+    o = q1;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->gpu_ms += ms;
+  return PPSK_OK;
+}
+
+void worker_main(ppsk_server* s, Worker* w) {
+  hipSetDevice(s->device);
+  for (;;) {
+    std::vector<ppsk_request*> batch;
+    int64_t rows = 0;
+    size_t off = 0;
+    StepLayout L{};
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      // admission: a free slot (this worker) and device memory for at least
+      // the oldest request
+      s->cv_work.wait(lk, [&] {
+        return s->stop ||
+               (!s->pending.empty() && s->arena.fits(layout(s, s->pending.front()->n_q).total));
+      });
+      if (s->stop) return;
+      // continuous batching: oldest first, while rows and memory allow
+      while (!s->pending.empty()) {
+        ppsk_request* r = s->pending.front();
+        const int64_t nr = rows + r->n_q;
+        if (nr > s->max_rows && rows > 0) break;
+        if (!s->arena.fits(layout(s, nr).total)) break;
+        batch.push_back(r);
+        rows = nr;
+        s->pending.pop_front();
+      }
+      L = layout(s, rows);
+      s->arena.alloc(L.total, &off);  // fits() held under the same lock
+      const double t = now_s(s->t0);
+      for (ppsk_request* r : batch) {
+        r->res.t_start = t;
+        r->res.step_rows = rows;
+        r->res.step_id = s->steps;
+      }
+      s->steps += 1;
+      s->inflight += 1;
+      s->peak_inflight = std::max(s->peak_inflight, s->inflight);
+    }
+    std::string err;
+    const int rc = run_step(s, *w, batch, rows, s->arena.base + off, L, err);
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->arena.release(off, L.total);
+      s->inflight -= 1;
+      const double t = now_s(s->t0);
+      for (ppsk_request* r : batch) {
+        r->status = rc;
+        r->error = err;
+        r->res.t_done = t;
+        r->done = true;
+      }
+      s->rows_done += rows;
+      s->requests_done += (int64_t)batch.size();
+    }
+    s->cv_done.notify_all();
+    s->cv_work.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ppsk_server_create(ppsk_set* ref, int32_t random_mode, double tolerance, int32_t kNN,
+                       int32_t knn_col, float cutoff, int32_t thr_col, int32_t max_rows,
+                       int32_t n_slots, size_t arena_bytes, ppsk_server** out) {
+  if (!out) return fail(PPSK_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (int rc = check_set(ref, "ref")) return rc;
+  if (ref->n_k < 2) return fail(PPSK_ERR_ARG, "core/accessory distances need at least two k-mer lengths");
+  if (random_mode != PPSK_RANDOM_NONE && random_mode != PPSK_RANDOM_EQUAL_BASE) {
+    return fail(PPSK_ERR_ARG, "serving supports no correction or the equal-base correction");
+  }
+  if (random_mode == PPSK_RANDOM_EQUAL_BASE && !ref->has_lengths) {
+    return fail(PPSK_ERR_ARG, "equal-base random correction needs sample lengths (ppsk_set_meta)");
+  }
+  if (kNN < 0 || kNN > ppsk::KNN_MAX || kNN > ref->n) return fail(PPSK_ERR_ARG, "kNN out of range");
+  if ((knn_col != 0 && knn_col != 1) || (thr_col != 0 && thr_col != 1)) {
+    return fail(PPSK_ERR_ARG, "dist_col out of range");
+  }
+  if (kNN == 0 && !(cutoff > 0)) return fail(PPSK_ERR_ARG, "nothing to return: kNN = 0 and cutoff <= 0");
+  if (max_rows < 1 || n_slots < 1 || n_slots > 16) {
+    return fail(PPSK_ERR_ARG, "max_rows >= 1 and n_slots in [1, 16] required");
+  }
+  DeviceCtx* c = nullptr;
+  if (int rc = get_ctx(ref->device, &c)) return rc;
+  auto* s = new ppsk_server();
+  s->ref = ref;
+  s->device = ref->device;
+  s->random_mode = random_mode;
+  s->tolerance = tolerance;
+  s->kNN = kNN;
+  s->knn_col = knn_col;
+  s->cutoff = cutoff;
+  s->thr_col = thr_col;
+  s->max_rows = max_rows;
+  s->variant = default_variant();
+  s->per_sample = (size_t)ref->n_k * ref->s64 * ppsk::BBITS * sizeof(uint64_t);
+  s->scratch_budget = (size_t)256 << 20;
+  if (arena_bytes == 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+    arena_bytes = fr > ((size_t)4 << 30) ? fr - ((size_t)4 << 30) : fr / 2;
+  }
+  const size_t one = layout(s, 1).total;
+  if (arena_bytes < one) {
+    delete s;
+    return fail(PPSK_ERR_MEMORY, "arena smaller than one single-query step (" +
+                                     std::to_string(one) + " bytes)");
+  }
+  hipError_t e = hipMalloc(&s->arena_mem, arena_bytes);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(PPSK_ERR_MEMORY, std::string("serving arena: ") + hipGetErrorString(e));
+  }
+  s->arena.init((char*)s->arena_mem, arena_bytes);
+  s->t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < n_slots; ++i) {
+    auto* w = new Worker();
+    if (hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&w->e0) != hipSuccess || hipEventCreate(&w->e1) != hipSuccess) {
+      delete w;
+      ppsk_server_free(s);
+      return fail(PPSK_ERR_HIP, "serving stream creation");
+    }
+    s->workers.emplace_back([s, w] {
+      worker_main(s, w);
+      hipStreamDestroy(w->st);
+      hipEventDestroy(w->e0);
+      hipEventDestroy(w->e1);
+      delete w;
+    });
+  }
+  *out = s;
+  return PPSK_OK;
+}
+
+int ppsk_server_submit(ppsk_server* s, int64_t n_queries, const uint64_t* raw,
+                       const uint64_t* lengths, int64_t* ticket) {
+  if (!s || !ticket) return fail(PPSK_ERR_ARG, "server / ticket is NULL");
+  if (n_queries < 1 || !raw) return fail(PPSK_ERR_ARG, "a request needs at least one query sketch");
+  if (n_queries > s->max_rows) {
+    return fail(PPSK_ERR_ARG, "request larger than max_rows (" + std::to_string(s->max_rows) + ")");
+  }
+  if (layout(s, n_queries).total > s->arena.size) {
+    return fail(PPSK_ERR_MEMORY, "request needs more device memory than the serving arena");
+  }
+  const ppsk_set* ref = s->ref;
+  auto* r = new ppsk_request();
+  r->n_q = n_queries;
+  r->raw.assign(raw, raw + (size_t)n_queries * (s->per_sample / sizeof(uint64_t)));
+  if (s->random_mode == PPSK_RANDOM_EQUAL_BASE) {
+    if (!lengths) {
+      delete r;
+      return fail(PPSK_ERR_ARG, "equal-base correction needs query lengths");
+    }
+    // csrs terms per query and k (src/random/random_match.cpp:318-336)
+    r->jv.resize((size_t)n_queries * ref->n_k);
+    for (int kk = 0; kk < ref->n_k; ++kk) {
+      const int64_t shift = (int64_t)(ref->klist[kk] - 1) * 2 + (ref->use_rc ? 1 : 0);
+      const double base = 1 - (double)1 / (double)((size_t)1 << shift);
+      for (int64_t q = 0; q < n_queries; ++q) {
+        r->jv[(size_t)q * ref->n_k + kk] = 1 - std::pow(base, (double)lengths[q]);
+      }
+    }
+  }
+  std::memset(&r->res, 0, sizeof(r->res));
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) {
+      delete r;
+      return fail(PPSK_ERR_STATE, "server stopped");
+    }
+    r->ticket = s->next_ticket++;
+    r->res.n_queries = n_queries;
+    r->res.t_submit = now_s(s->t0);
+    s->tickets[r->ticket] = r;
+    s->pending.push_back(r);
+    *ticket = r->ticket;
+  }
+  s->cv_work.notify_one();
+  return PPSK_OK;
+}
+
+int ppsk_server_wait(ppsk_server* s, int64_t ticket, ppsk_result** out) {
+  if (!s || !out) return fail(PPSK_ERR_ARG, "server / out is NULL");
+  *out = nullptr;
+  ppsk_request* r = nullptr;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(PPSK_ERR_ARG, "unknown or already collected ticket");
+    r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    s->tickets.erase(it);
+  }
+  if (r->status != PPSK_OK) {
+    const int rc = r->status;
+    const std::string msg = r->error;
+    delete r;
+    return fail(rc, "serving step failed: " + msg);
+  }
+  ppsk_result& res = r->res;
+  res.knn_n = (int64_t)r->kj.size();
+  res.knn_i = r->ki.data();
+  res.knn_j = r->kj.data();
+  res.knn_d = r->kd.data();
+  res.thr_n = (int64_t)r->tj.size();
+  res.thr_i = r->ti.data();
+  res.thr_j = r->tj.data();
+  res.thr_d = r->td.data();
+  res.owner = r;
+  *out = &res;
+  return PPSK_OK;
+}
+
+void ppsk_result_free(ppsk_result* res) {
+  if (res) delete (ppsk_request*)res->owner;
+}
+
+int ppsk_server_stats(ppsk_server* s, int64_t* steps, int64_t* rows, int64_t* requests,
+                      int32_t* peak_inflight, size_t* peak_arena_bytes, double* gpu_ms) {
+  if (!s) return fail(PPSK_ERR_ARG, "server is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (steps) *steps = s->steps;
+  if (rows) *rows = s->rows_done;
+  if (requests) *requests = s->requests_done;
+  if (peak_inflight) *peak_inflight = s->peak_inflight;
+  if (peak_arena_bytes) *peak_arena_bytes = s->arena.peak;
+  if (gpu_ms) *gpu_ms = s->gpu_ms;
+  return PPSK_OK;
+}
+
+int ppsk_server_free(ppsk_server* s) {
+  if (!s) return PPSK_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (auto& t : s->workers) t.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (ppsk_request* r : s->pending) {
+      r->status = PPSK_ERR_STATE;
+      r->error = "server stopped";
+      r->done = true;
+    }
+    s->pending.clear();
+    for (auto& kv : s->tickets) delete kv.second;
+    s->tickets.clear();
+  }
+  s->cv_done.notify_all();
+  if (s->arena_mem) {
+    hipSetDevice(s->device);
+    hipFree(s->arena_mem);
+  }
+  delete s;
+  return PPSK_OK;
+}
+
+}  // extern "C"

@@ -29,8 +29,28 @@ EXPORTS = [
     "ppsk_set_upload", "ppsk_set_meta", "ppsk_set_free", "ppsk_set_size", "ppsk_dense",
     "ppsk_knn", "ppsk_threshold", "ppsk_self_pass", "ppsk_free_host", "ppsk_host_alloc",
     "ppsk_host_free", "ppsk_device_alloc", "ppsk_device_free", "ppsk_copy_to_host",
-    "ppsk_last_timing", "ppsk_synth_sketches",
+    "ppsk_last_timing", "ppsk_synth_sketches", "ppsk_set_prefix", "ppsk_batch_create",
+    "ppsk_batch_run", "ppsk_batch_tuning", "ppsk_batch_free", "ppsk_server_create",
+    "ppsk_server_submit", "ppsk_server_wait", "ppsk_result_free", "ppsk_server_stats",
+    "ppsk_server_free",
 ]
+
+
+class Job(_c.Structure):
+    """ppsk_job (include/ppsk.h)."""
+    _fields_ = [("ref", _p), ("query", _p), ("random_mode", _i32), ("n_clusters", _i32),
+                ("mc_table", _p), ("tolerance", _c.c_double), ("out_kind", _i32),
+                ("out", _p)]
+
+
+class Result(_c.Structure):
+    """ppsk_result (include/ppsk.h)."""
+    _fields_ = [("n_queries", _i64), ("knn_n", _i64), ("knn_i", _c.POINTER(_i64)),
+                ("knn_j", _c.POINTER(_i64)), ("knn_d", _c.POINTER(_c.c_float)),
+                ("thr_n", _i64), ("thr_i", _c.POINTER(_i64)), ("thr_j", _c.POINTER(_i64)),
+                ("thr_d", _c.POINTER(_c.c_float)), ("n_fail", _u64), ("t_submit", _c.c_double),
+                ("t_start", _c.c_double), ("t_done", _c.c_double), ("step_id", _i64),
+                ("step_rows", _i64), ("owner", _p)]
 
 
 class PpskError(RuntimeError):
@@ -79,6 +99,21 @@ def lib():
     L.ppsk_last_timing.argtypes = [_c.POINTER(_c.c_double), _c.POINTER(_i64), _c.POINTER(_i64)]
     L.ppsk_synth_sketches.argtypes = [_i64, _i32, _p, _i32, _u64, _c.c_double, _c.c_double,
                                       _c.c_double, _p, _p, _p, _i32]
+    L.ppsk_set_prefix.argtypes = [_p, _i64, _c.POINTER(_p)]
+    L.ppsk_batch_create.argtypes = [_c.c_int, _c.POINTER(Job), _i32, _c.POINTER(_p)]
+    L.ppsk_batch_run.argtypes = [_p, _i32, _p, _c.POINTER(_i32), _c.POINTER(_c.c_double)]
+    L.ppsk_batch_tuning.argtypes = [_p, _c.POINTER(_i32), _p]
+    L.ppsk_batch_free.argtypes = [_p]
+    L.ppsk_server_create.argtypes = [_p, _i32, _c.c_double, _i32, _i32, _c.c_float, _i32, _i32,
+                                     _i32, _c.c_size_t, _c.POINTER(_p)]
+    L.ppsk_server_submit.argtypes = [_p, _i64, _p, _p, _c.POINTER(_i64)]
+    L.ppsk_server_wait.argtypes = [_p, _i64, _c.POINTER(_c.POINTER(Result))]
+    L.ppsk_result_free.argtypes = [_c.POINTER(Result)]
+    L.ppsk_result_free.restype = None
+    L.ppsk_server_stats.argtypes = [_p, _c.POINTER(_i64), _c.POINTER(_i64), _c.POINTER(_i64),
+                                    _c.POINTER(_i32), _c.POINTER(_c.c_size_t),
+                                    _c.POINTER(_c.c_double)]
+    L.ppsk_server_free.argtypes = [_p]
     _lib = L
     return L
 
@@ -148,6 +183,10 @@ class SketchSet:
             raise PpskError(ERR_STATE, "sketch set freed")
         return self._h
 
+    def prefix(self, count):
+        """Read-only view of the first `count` samples (no copy)."""
+        return SetView(self, count)
+
     def free(self):
         if getattr(self, "_h", None) is not None:
             lib().ppsk_set_free(self._h)
@@ -164,6 +203,171 @@ class SketchSet:
             self.free()
         except Exception:
             pass
+
+
+class SetView(SketchSet):
+    """ppsk_set_prefix view; keeps its parent alive."""
+
+    def __init__(self, parent, count):
+        h = _p()
+        _check(lib().ppsk_set_prefix(parent.handle, int(count), _c.byref(h)))
+        self._h = h
+        self.parent = parent
+        self.n = int(count)
+        self.klist = parent.klist
+        self.s64 = parent.s64
+        self.device = parent.device
+
+
+class Batch:
+    """A batch of query-vs-ref jobs run by the persistent kernel.
+
+    jobs: list of dicts with keys ref, query (SketchSet), and optionally
+    random_mode, table, tol, out ("core_acc" | "jaccard" | "counts").  Each
+    job's output lives in device memory; results() copies them back."""
+
+    def __init__(self, jobs, device=0):
+        self.device = device
+        self._keep = []
+        self.specs = []
+        arr = (Job * len(jobs))()
+        for x, j in enumerate(jobs):
+            ref, query = j["ref"], j["query"]
+            kind = OUT_KINDS[j.get("out", "core_acc")]
+            mode, ncl, tab = _random_args(j.get("random_mode", RANDOM_NONE), j.get("table"))
+            tol = j.get("tol")
+            tol = default_tolerance(ref.s64) if tol is None else tol
+            rows = query.n * ref.n
+            n_k = len(ref.klist)
+            if kind == 0:
+                shape, dt = (rows, 2), np.float32
+            elif kind == 1:
+                shape, dt = (rows, n_k), np.float32
+            else:
+                shape, dt = (rows, n_k), np.uint16
+            nbytes = int(np.prod(shape)) * np.dtype(dt).itemsize
+            dptr = device_alloc(nbytes, device)
+            self.specs.append((dptr, shape, dt))
+            self._keep += [ref, query, tab]
+            arr[x] = Job(ref.handle, query.handle, mode, ncl, _ptr(tab), tol, kind, dptr)
+        self._jobs = arr
+        h = _p()
+        try:
+            _check(lib().ppsk_batch_create(device, arr, len(jobs), _c.byref(h)))
+        except Exception:
+            self._free_outputs()
+            raise
+        self._h = h
+
+    def run(self, variant=0):
+        """Run every job.  Returns (n_fail per job, variant used, kernel ms)."""
+        nf = np.zeros(len(self.specs), np.uint64)
+        used = _i32()
+        ms = _c.c_double()
+        _check(lib().ppsk_batch_run(self._h, variant, _ptr(nf), _c.byref(used), _c.byref(ms)),
+               allow=(ERR_FIT,))
+        return nf, used.value, ms.value
+
+    def tuning(self):
+        v = _i32()
+        ms = np.zeros(8, np.float64)
+        _check(lib().ppsk_batch_tuning(self._h, _c.byref(v), _ptr(ms)))
+        return v.value, ms
+
+    def result(self, x):
+        dptr, shape, dt = self.specs[x]
+        out = np.empty(shape, dt)
+        copy_to_host(out, dptr)
+        return out
+
+    def _free_outputs(self):
+        for dptr, _, _ in self.specs:
+            device_free(dptr)
+        self.specs = []
+
+    def free(self):
+        if getattr(self, "_h", None) is not None:
+            lib().ppsk_batch_free(self._h)
+            self._h = None
+        if getattr(self, "specs", None):
+            self._free_outputs()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
+
+
+def _copy_out(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(ptr, (n,)).copy()
+
+
+class Server:
+    """Serving loop over a resident reference set (ppsk_server_*).
+
+    submit() queues a query batch and returns a ticket; wait(ticket) blocks
+    (outside the GIL) and returns a dict with the kNN and threshold lists and
+    the request's timestamps."""
+
+    def __init__(self, ref, kNN=10, knn_col=0, cutoff=0.05, thr_col=0,
+                 random_mode=RANDOM_NONE, tol=None, max_rows=1024, n_slots=4, arena_bytes=0):
+        self.ref = ref
+        self.random_mode = random_mode
+        tol = default_tolerance(ref.s64) if tol is None else tol
+        h = _p()
+        _check(lib().ppsk_server_create(ref.handle, random_mode, tol, kNN, knn_col, cutoff,
+                                        thr_col, max_rows, n_slots, arena_bytes, _c.byref(h)))
+        self._h = h
+
+    def submit(self, raw, lengths=None):
+        raw = np.ascontiguousarray(raw, np.uint64)
+        le = None if lengths is None else np.ascontiguousarray(lengths, np.uint64)
+        t = _i64()
+        _check(lib().ppsk_server_submit(self._h, raw.shape[0], _ptr(raw), _ptr(le),
+                                        _c.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        r = _c.POINTER(Result)()
+        _check(lib().ppsk_server_wait(self._h, ticket, _c.byref(r)))
+        try:
+            x = r.contents
+            out = dict(
+                knn_i=_copy_out(x.knn_i, x.knn_n, np.int64),
+                knn_j=_copy_out(x.knn_j, x.knn_n, np.int64),
+                knn_d=_copy_out(x.knn_d, x.knn_n, np.float32),
+                thr_i=_copy_out(x.thr_i, x.thr_n, np.int64),
+                thr_j=_copy_out(x.thr_j, x.thr_n, np.int64),
+                thr_d=_copy_out(x.thr_d, x.thr_n, np.float32),
+                n_fail=x.n_fail, t_submit=x.t_submit, t_start=x.t_start, t_done=x.t_done,
+                step_id=x.step_id, step_rows=x.step_rows, n_queries=x.n_queries)
+        finally:
+            lib().ppsk_result_free(r)
+        return out
+
+    def stats(self):
+        st, rows, req = _i64(), _i64(), _i64()
+        pk = _i32()
+        pa = _c.c_size_t()
+        ms = _c.c_double()
+        _check(lib().ppsk_server_stats(self._h, _c.byref(st), _c.byref(rows), _c.byref(req),
+                                       _c.byref(pk), _c.byref(pa), _c.byref(ms)))
+        return dict(steps=st.value, rows=rows.value, requests=req.value,
+                    peak_inflight=pk.value, peak_arena_bytes=pa.value, gpu_ms=ms.value)
+
+    def free(self):
+        if getattr(self, "_h", None) is not None:
+            lib().ppsk_server_free(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
 
 
 def _random_args(random_mode, table):

@@ -1,0 +1,137 @@
+"""Serving loop: a resident reference set answers query batches with kNN and
+threshold lists under continuous batching.
+
+* A request's results are the same served alone and served in a mix with
+  many concurrent requests (sharing steps with them), bit for bit.
+* Results equal the plain path: ppsk_dense in query mode, then the
+  reference heap (kNN, no sample excluded) and a row-major threshold scan
+  over the same dense rows; the dense rows agree with the oracle.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+import ppsk_hip as H
+from helpers import KLIST8
+
+pytestmark = pytest.mark.gpu
+
+N_REF = 3000
+N_Q = 700
+S64 = 156
+KNN = 10
+CUTOFF = 0.05
+
+
+@pytest.fixture(scope="module")
+def data():
+    raw, le, _ = H.synth(N_REF + N_Q, KLIST8, S64, seed=17)
+    ref_raw, ref_le = raw[:N_REF].copy(), le[:N_REF].copy()
+    q_raw, q_le = raw[N_REF:].copy(), le[N_REF:].copy()
+    ref = H.SketchSet(ref_raw, KLIST8, S64, lengths=ref_le)
+    yield dict(ref=ref, ref_raw=ref_raw, ref_le=ref_le, q_raw=q_raw, q_le=q_le)
+    ref.free()
+
+
+def _plain(ref, q_raw, q_le):
+    """The plain path for one request: dense rows, heap kNN, threshold scan."""
+    with H.SketchSet(q_raw, KLIST8, S64, lengths=q_le) as q:
+        d, nf, _ = H.dense(ref, q, random_mode=H.RANDOM_EQUAL_BASE)
+    nq = q_raw.shape[0]
+    core = d[:, 0].reshape(nq, N_REF)
+    ki, kj, kd = O.knn_rows(core, np.full(nq, -1), KNN)
+    ti, tj = np.nonzero(core < CUTOFF)
+    td = core[ti, tj]
+    return d, nf, (ki, kj, kd), (ti.astype(np.int64), tj.astype(np.int64), td)
+
+
+def _same(a, b):
+    for k in ("knn_i", "knn_j", "thr_i", "thr_j"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("knn_d", "thr_d"):
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+    assert a["n_fail"] == b["n_fail"]
+
+
+def test_request_same_alone_and_in_mix(data):
+    ref, q_raw, q_le = data["ref"], data["q_raw"], data["q_le"]
+    a0, a1 = 100, 137  # the request under test: 37 queries
+    with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
+                  max_rows=256, n_slots=3) as srv:
+        alone = srv.wait(srv.submit(q_raw[a0:a1], q_le[a0:a1]))
+        assert alone["step_rows"] == a1 - a0
+        # the mix: 40 requests of 1..64 queries from 4 submitting threads,
+        # the request under test among them
+        rng = np.random.default_rng(5)
+        spans = []
+        for _ in range(40):
+            s = int(rng.integers(0, N_Q - 64))
+            spans.append((s, s + int(rng.integers(1, 65))))
+        spans.insert(17, (a0, a1))
+        tickets = [None] * len(spans)
+
+        def submit(part):
+            for x in part:
+                s, e = spans[x]
+                tickets[x] = srv.submit(q_raw[s:e], q_le[s:e])
+
+        parts = [list(range(t, len(spans), 4)) for t in range(4)]
+        th = [threading.Thread(target=submit, args=(p,)) for p in parts]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    mixed = res[17]
+    _same(alone, mixed)
+    # continuous batching happened: some steps carried several requests
+    assert st["steps"] < st["requests"]
+    assert max(r["step_rows"] for r in res) > max(e - s for s, e in spans)
+    # every request in the mix equals its own plain-path result
+    for x in (0, 17, 39):
+        s, e = spans[x]
+        _, _, (ki, kj, kd), (ti, tj, td) = _plain(ref, q_raw[s:e], q_le[s:e])
+        r = res[x]
+        assert np.array_equal(r["knn_i"], ki) and np.array_equal(r["knn_j"], kj)
+        assert np.array_equal(r["knn_d"].view(np.uint32), kd.view(np.uint32))
+        assert np.array_equal(r["thr_i"], ti) and np.array_equal(r["thr_j"], tj)
+        assert np.array_equal(r["thr_d"].view(np.uint32), td.view(np.uint32))
+
+
+def test_server_against_oracle(data):
+    ref, q_raw, q_le = data["ref"], data["q_raw"], data["q_le"]
+    s, e = 300, 320
+    with H.Server(ref, kNN=KNN, knn_col=1, cutoff=0.3, thr_col=1,
+                  random_mode=H.RANDOM_EQUAL_BASE, max_rows=64, n_slots=2) as srv:
+        r = srv.wait(srv.submit(q_raw[s:e], q_le[s:e]))
+    want, _, nf, _ = O.query_db(data["ref_raw"], KLIST8, S64, 14, data["ref_le"],
+                                query=q_raw[s:e], query_len=q_le[s:e],
+                                random=O.Random(1, True))
+    assert nf == 0 and r["n_fail"] == 0
+    acc = want[:, 1].reshape(e - s, N_REF)
+    ki, kj, kd = O.knn_rows(acc, np.full(e - s, -1), KNN)
+    assert np.array_equal(r["knn_i"], ki)
+    # distances within the core/accessory tolerance; neighbours exact unless
+    # two candidates are within that tolerance of each other
+    assert np.abs(r["knn_d"] - kd).max() <= 1e-6
+    got_d = acc[r["knn_i"], r["knn_j"]]
+    assert np.abs(got_d - r["knn_d"]).max() <= 1e-6
+    ti, tj = np.nonzero(acc < 0.3)
+    sel = np.abs(acc[ti, tj] - 0.3) > 1e-6  # away from the cutoff edge
+    got = set(zip(r["thr_i"].tolist(), r["thr_j"].tolist()))
+    assert set(zip(ti[sel].tolist(), tj[sel].tolist())) <= got
+
+
+def test_server_rejects_bad_requests(data):
+    ref, q_raw, q_le = data["ref"], data["q_raw"], data["q_le"]
+    with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
+                  max_rows=16, n_slots=1) as srv:
+        with pytest.raises(H.PpskError, match="max_rows"):
+            srv.submit(q_raw[:17], q_le[:17])
+        with pytest.raises(H.PpskError, match="lengths"):
+            srv.submit(q_raw[:2])
+        with pytest.raises(H.PpskError, match="ticket"):
+            srv.wait(12345)

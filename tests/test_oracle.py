@@ -297,3 +297,25 @@ def test_knn_heap_tie_order_is_libstdcxx():
     for r in range(n):
         row = j[r * 4:(r + 1) * 4]
         assert r not in row and len(set(row.tolist())) == 4
+
+
+def test_knn_rows_matches_square_heap():
+    """knn_rows over a subset of rows of a square (own column excluded) is the
+    square heap restricted to those rows, ties included; with no exclusion
+    and distinct distances it is a plain ascending sort."""
+    core = _gold("ppsketch_ref")[:, 0]
+    square = O.long_to_square(core)
+    square[3, :5] = square[3, 5]  # a run of ties in one row
+    n = square.shape[0]
+    sel = np.array([0, 3, 7, n - 1])
+    for knn in (1, 4, n):
+        i, j, d = O.knn_from_square(square, knn)
+        ri, rj, rd = O.knn_rows(square[sel], sel, knn)
+        full = np.arange(n * knn).reshape(n, knn)[sel].ravel()
+        assert np.array_equal(ri, i[full]) and np.array_equal(rj, j[full])
+        assert np.array_equal(rd.view(np.uint32), d[full].view(np.uint32))
+    rng = np.random.default_rng(3)
+    rows = rng.permutation(40 * 25).astype(np.float32).reshape(40, 25)
+    ri, rj, rd = O.knn_rows(rows, np.full(40, -1), 6)
+    assert np.array_equal(ri, np.repeat(np.arange(40), 6))
+    assert np.array_equal(rj.reshape(40, 6), np.argsort(rows, axis=1)[:, :6])
