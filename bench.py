@@ -47,6 +47,16 @@ def parse():
                     help="samples of the CPU baseline's all-vs-all sample")
     ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                     help="run one plain copy kernel of BYTES and exit (HBM counter calibration)")
+    ap.add_argument("--workload", default="plain", choices=["plain", "large", "batch", "serve"],
+                    help="plain: 8,192-sample all-vs-all (the headline line); large: "
+                         "32,768-sample self pass streamed to host + kNN; batch: 256 uneven "
+                         "query-vs-ref jobs; serve: resident 200,000-sample database under "
+                         "Poisson query batches")
+    ap.add_argument("--serve-refs", type=int, default=200_000)
+    ap.add_argument("--serve-qps", type=float, default=0.0,
+                    help="Poisson arrival rate (0: 70%% of the measured saturation rate)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--batch-jobs", type=int, default=256)
     return ap.parse_args()
 
 
@@ -120,6 +130,11 @@ def main():
         sys.exit(2)
     if a.copy_only:
         copy_only(a.copy_only)
+        return
+    if a.workload != "plain":
+        import bench_workloads as W
+        line = {"large": W.large, "batch": W.batch, "serve": W.serve}[a.workload](a)
+        print(json.dumps(line), flush=True)
         return
     import ppsk_hip as H
 

@@ -188,7 +188,8 @@ int ppsk_copy_to_host(void *dst, const void *device_src, size_t bytes);
  * ppsk_dense call in query mode (rows = query samples, query-major output,
  * r1 = query) with its own sets, k-mer lengths, sketch size, random
  * correction, tolerance and device output buffer (ppsk_device_alloc, sized
- * as for ppsk_dense).  The reference runs such jobs one queryDatabase /
+ * as for ppsk_dense); a job whose query is its ref (the same handle) is a
+ * self job (condensed upper triangle, as ppsk_dense with query == NULL).  The reference runs such jobs one queryDatabase /
  * query_db_cuda call at a time (src/gpu/gpu_api.cpp:352-452); results here
  * are identical to those single calls.
  */
@@ -214,7 +215,7 @@ int ppsk_batch_create(int device_id, const ppsk_job *jobs, int32_t n_jobs,
 
 /*
  * Run every job: one graph launch (queue reset + persistent kernel), replayed
- * on later runs.  variant: tile shape (2: 64 x 256, 6: 32 x 256) or 0 for the
+ * on later runs.  variant: tile shape (2: 64 x 256, 4: 32 x 256) or 0 for the
  * autotuner, which on the first run of a batch with an unseen job-shape
  * signature times each shape on the batch itself and keeps the fastest for
  * the process.  n_fail (may be NULL): per-job count of pairs whose regression
@@ -224,7 +225,7 @@ int ppsk_batch_create(int device_id, const ppsk_job *jobs, int32_t n_jobs,
 int ppsk_batch_run(ppsk_batch *batch, int32_t variant, uint64_t *n_fail,
                    int32_t *used_variant, double *kernel_ms);
 
-/* Autotuner result: chosen shape and per-shape time (index = shape, 0..6). */
+/* Autotuner result: chosen shape and per-shape time (index = shape, 0..5). */
 int ppsk_batch_tuning(const ppsk_batch *batch, int32_t *variant,
                       double *ms_per_variant);
 

@@ -1,0 +1,274 @@
+"""Secondary benchmark workloads of bench.py (--workload large | batch | serve).
+
+Each returns one JSON-ready dict.  All inputs are seeded synthetic sketches
+from the library's tree generator, resident in HBM before timing starts.
+
+large   32,768 samples, 8 k-mer lengths, 10,000 bins: one self pass over
+        536,854,528 pairs, dense core/accessory streamed to page-locked host
+        memory in row blocks, plus kNN = 10 lists.
+batch   256 query-vs-ref jobs of uneven shape (queries log-uniform in
+        [1, 2048], references uniform in [500, 20000], 4-14 k-mer lengths
+        within k = 13..39, 1,000-20,000 bins), no random-match correction
+        (the equal-base form is undefined for k >= 33).  Jobs use prefixes of
+        8 resident databases, one per (k-set, sketch size) class.  Timed as
+        one persistent launch (graph replay, autotuned tile shape) and as the
+        plain path: one ppsk_dense call per job.
+serve   a resident database (default 200,000 samples, 8 k, 10,000 bins,
+        about 28 GB) serving query batches of log-uniform size in [1, 512]
+        with kNN = 10 and threshold 0.05 (core), equal-base correction.
+        Saturation throughput (every request queued at once), then Poisson
+        arrivals at a fixed rate for latency percentiles, and the plain path
+        (each request alone: one query-vs-ref ppsk_dense into device memory).
+"""
+import json
+import math
+import os
+import threading
+import time
+
+import numpy as np
+
+import ppsk_hip as H
+
+KL8 = list(range(15, 30, 2))
+S64 = 156
+
+
+def _ops_per_pair(n_k, s64):
+    return n_k * s64 * 30
+
+
+# ---------------------------------------------------------------------------
+def large(a):
+    n = 32768
+    t0 = time.time()
+    raw, le, _ = H.synth(n, KL8, S64, seed=3)
+    print(f"# generated {n} sketches in {time.time() - t0:.1f}s", flush=True)
+    pairs = n * (n - 1) // 2
+    with H.SketchSet(raw, KL8, S64, lengths=le) as s, \
+            H.PinnedBuffer((pairs, 2), np.float32) as pin:
+        H.self_pass(s, pin.array, kNN=10, random_mode=H.RANDOM_EQUAL_BASE)  # warm-up
+        walls, kms = [], []
+        for _ in range(max(1, a.steps // 5)):
+            t = time.perf_counter()
+            _, _, nf = H.self_pass(s, pin.array, kNN=10, random_mode=H.RANDOM_EQUAL_BASE)
+            walls.append(time.perf_counter() - t)
+            kms.append(H.last_timing()[0])
+            if nf:
+                raise RuntimeError(f"{nf} pairs failed the k-mer regression")
+    wall = min(walls)
+    ms = min(kms)
+    return {
+        "metric": "distances/sec, 32,768-sample all-vs-all streamed to host + kNN=10",
+        "value": round(pairs / wall, 1), "unit": "distances/s", "n_gpus": 1,
+        "wall_s": round(wall, 4), "dist_kernel_ms": round(ms, 2),
+        "valu_frac": round(pairs * _ops_per_pair(8, S64) / (ms * 1e-3) / 39.3216e12, 4),
+        "dense_bytes_to_host": pairs * 8,
+        "host_rate_GBps": round(pairs * 8 / wall / 1e9, 2),
+        "config": {"workload": "large all-vs-all, dense FP32 to pinned host + kNN=10",
+                   "n_samples": n, "pairs": pairs, "kmers": KL8, "bins": S64 * 64},
+    }
+
+
+# ---------------------------------------------------------------------------
+def _batch_jobs(n_jobs, rng):
+    jobs = []
+    for _ in range(n_jobs):
+        q = int(math.floor(2048.0 ** rng.uniform()))
+        q = min(max(q, 1), 2048)
+        r = int(rng.integers(500, 20001))
+        jobs.append((q, r))
+    return jobs
+
+
+def batch(a):
+    rng = np.random.default_rng(11)
+    n_classes = 8
+    classes = []
+    for c in range(n_classes):
+        k = int(rng.integers(4, 15))
+        kl = sorted(rng.choice(np.arange(13, 40), size=k, replace=False).tolist())
+        s64 = int(round(rng.uniform(1000, 20000) / 64))
+        classes.append((kl, s64))
+    shapes = _batch_jobs(a.batch_jobs, rng)
+    per_class = [[] for _ in range(n_classes)]
+    for x, (q, r) in enumerate(shapes):
+        per_class[x % n_classes].append((x, q, r))
+    t0 = time.time()
+    dbs = []
+    for c, (kl, s64) in enumerate(classes):
+        rmax = max(r for _, _, r in per_class[c])
+        qmax = max(q for _, q, _ in per_class[c])
+        raw, _, _ = H.synth(rmax + qmax, kl, s64, seed=500 + c)
+        ref = H.SketchSet(raw[:rmax], kl, s64)
+        qry = H.SketchSet(raw[rmax:], kl, s64)
+        del raw
+        dbs.append((ref, qry))
+    print(f"# generated and uploaded {n_classes} databases in {time.time() - t0:.1f}s",
+          flush=True)
+    jobs, views, pairs, ops = [], [], 0, 0
+    for c in range(n_classes):
+        kl, s64 = classes[c]
+        ref, qry = dbs[c]
+        for x, q, r in per_class[c]:
+            rv, qv = ref.prefix(r), qry.prefix(q)
+            views.append((rv, qv))
+            jobs.append(dict(ref=rv, query=qv, random_mode=H.RANDOM_NONE))
+            pairs += q * r
+            ops += q * r * _ops_per_pair(len(kl), s64)
+    b = H.Batch(jobs)
+    t = time.perf_counter()
+    b.run()  # first run: the autotuner times each tile shape on this batch
+    tune_s = time.perf_counter() - t
+    chosen, per_shape = b.tuning()
+    times, kms = [], []
+    for _ in range(max(1, a.steps // 2)):
+        t = time.perf_counter()
+        nf, used, ms = b.run()
+        times.append(time.perf_counter() - t)
+        kms.append(ms)
+    batch_s = min(times)
+    # plain path: one ppsk_dense call per job, output in device memory
+    plain_s = 0.0
+    for rv, qv in views:
+        d = H.device_alloc(qv.n * rv.n * 8)
+        t = time.perf_counter()
+        H.dense_device(rv, qv, dptr=d)
+        plain_s += time.perf_counter() - t
+        H.device_free(d)
+    b.free()
+    for rv, qv in views:
+        rv.free()
+        qv.free()
+    for ref, qry in dbs:
+        ref.free()
+        qry.free()
+    return {
+        "metric": "distances/sec, 256 uneven query-vs-ref jobs in one persistent launch",
+        "value": round(pairs / batch_s, 1), "unit": "distances/s", "n_gpus": 1,
+        "batch_s": round(batch_s, 4), "batch_kernel_ms": round(min(kms), 2),
+        "plain_value": round(pairs / plain_s, 1), "plain_s": round(plain_s, 4),
+        "speedup_vs_plain": round(plain_s / batch_s, 2),
+        "valu_frac": round(ops / (min(kms) * 1e-3) / 39.3216e12, 4),
+        "autotune": {"chosen_shape": chosen, "first_run_s": round(tune_s, 3),
+                     "shape_ms": {str(v): round(float(per_shape[v]), 2) for v in (2, 4)}},
+        "config": {"workload": "batched uneven query-vs-ref jobs", "jobs": len(jobs),
+                   "pairs": pairs, "queries": "log-uniform 1..2048",
+                   "refs": "uniform 500..20000", "classes": [
+                       {"kmers": kl, "bins": s64 * 64} for kl, s64 in classes],
+                   "random_correction": "none"},
+    }
+
+
+# ---------------------------------------------------------------------------
+def serve(a):
+    n_ref = a.serve_refs
+    n_pool = 4096
+    t0 = time.time()
+    # a wider tree than the all-vs-all workloads (core divergence up to 15%
+    # root to leaf) so that the 0.05 threshold keeps close relatives only,
+    # as in a real database
+    raw, le, _ = H.synth(n_ref + n_pool, KL8, S64, seed=21, dcore_max=0.15)
+    q_raw, q_le = raw[n_ref:].copy(), le[n_ref:].copy()
+    ref = H.SketchSet(raw[:n_ref], KL8, S64, lengths=le[:n_ref])
+    del raw
+    print(f"# generated and uploaded {n_ref} reference sketches in {time.time() - t0:.1f}s",
+          flush=True)
+    rng = np.random.default_rng(4)
+
+    def draw(m):
+        out = []
+        for _ in range(m):
+            size = min(512, max(1, int(math.floor(512.0 ** rng.uniform()))))
+            s = int(rng.integers(0, n_pool - size + 1))
+            out.append((s, s + size))
+        return out
+
+    # unbatched: the same server, one request at a time (submit, wait)
+    solo_req = draw(32)
+    with H.Server(ref, kNN=10, cutoff=0.05, random_mode=H.RANDOM_EQUAL_BASE,
+                  max_rows=512, n_slots=1) as solo:
+        t = time.perf_counter()
+        for s_, e_ in solo_req:
+            solo.wait(solo.submit(q_raw[s_:e_], q_le[s_:e_]))
+        solo_s = time.perf_counter() - t
+    solo_rate = sum(e_ - s_ for s_, e_ in solo_req) * n_ref / solo_s
+
+    # plain path: each request alone, one query-vs-ref dense launch
+    plain_req = draw(48)
+    plain_pairs, plain_s = 0, 0.0
+    for s_, e_ in plain_req:
+        with H.SketchSet(q_raw[s_:e_], KL8, S64, lengths=q_le[s_:e_]) as q:
+            d = H.device_alloc((e_ - s_) * n_ref * 8)
+            t = time.perf_counter()
+            H.dense_device(ref, q, random_mode=H.RANDOM_EQUAL_BASE, dptr=d)
+            plain_s += time.perf_counter() - t
+            H.device_free(d)
+        plain_pairs += (e_ - s_) * n_ref
+    plain_rate = plain_pairs / plain_s
+
+    srv = H.Server(ref, kNN=10, cutoff=0.05, random_mode=H.RANDOM_EQUAL_BASE,
+                   max_rows=1024, n_slots=4)
+    try:
+        # saturation: every request queued at once
+        sat_req = draw(64)
+        t = time.perf_counter()
+        tickets = [srv.submit(q_raw[s_:e_], q_le[s_:e_]) for s_, e_ in sat_req]
+        sat_res = [srv.wait(tk) for tk in tickets]
+        thr_per_query = sum(len(r["thr_j"]) for r in sat_res) / sum(
+            e_ - s_ for s_, e_ in sat_req)
+        sat_s = time.perf_counter() - t
+        sat_pairs = sum(e_ - s_ for s_, e_ in sat_req) * n_ref
+        sat_rate = sat_pairs / sat_s
+        sat_qps = len(sat_req) / sat_s
+        # Poisson arrivals at a fixed rate
+        qps = a.serve_qps or 0.7 * sat_qps
+        n_req = max(8, int(qps * a.serve_seconds))
+        reqs = draw(n_req)
+        gaps = rng.exponential(1.0 / qps, size=n_req)
+        tickets = [None] * n_req
+        results = [None] * n_req
+
+        def consumer():
+            for x in range(n_req):
+                while tickets[x] is None:
+                    time.sleep(0.0005)
+                results[x] = srv.wait(tickets[x])
+
+        th = threading.Thread(target=consumer)
+        th.start()
+        t_begin = time.perf_counter()
+        nxt = t_begin
+        for x, (s_, e_) in enumerate(reqs):
+            nxt += gaps[x]
+            dt = nxt - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets[x] = srv.submit(q_raw[s_:e_], q_le[s_:e_])
+        th.join()
+        wall = time.perf_counter() - t_begin
+        st = srv.stats()
+    finally:
+        srv.free()
+        ref.free()
+    lat = np.array([(r["t_done"] - r["t_submit"]) * 1e3 for r in results])
+    pairs = sum(e_ - s_ for s_, e_ in reqs) * n_ref
+    return {
+        "metric": "served distances/sec with p50/p99 request latency (kNN=10, threshold 0.05)",
+        "value": round(pairs / wall, 1), "unit": "distances/s", "n_gpus": 1,
+        "p50_ms": round(float(np.percentile(lat, 50)), 2),
+        "p99_ms": round(float(np.percentile(lat, 99)), 2),
+        "qps": round(qps, 3), "requests": n_req, "wall_s": round(wall, 3),
+        "saturation_value": round(sat_rate, 1), "saturation_qps": round(sat_qps, 3),
+        "plain_value": round(plain_rate, 1),
+        "speedup_vs_plain": round(sat_rate / plain_rate, 2),
+        "unbatched_value": round(solo_rate, 1),
+        "speedup_vs_unbatched": round(sat_rate / solo_rate, 2),
+        "threshold_hits_per_query": round(thr_per_query, 1),
+        "n_fail": int(sum(r["n_fail"] for r in results)),
+        "server": st,
+        "config": {"workload": "serving, resident reference database", "n_refs": n_ref,
+                   "kmers": KL8, "bins": S64 * 64, "request_sizes": "log-uniform 1..512",
+                   "kNN": 10, "threshold": 0.05, "slots": 4, "max_rows_per_step": 1024,
+                   "random_correction": "equal-base"},
+    }

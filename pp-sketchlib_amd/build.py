@@ -47,13 +47,44 @@ def _run(cmd):
         raise RuntimeError("build step failed: " + " ".join(cmd))
 
 
+def _check_pipelined_spills(remarks):
+    """Reject a build in which a pipelined tile configuration spills.
+
+    The PB = 2 matching loop issues scalar loads from inline asm and waits for
+    them later; if the compiler spilled or copied one of those SGPR tuples in
+    between, it would move data that has not arrived yet.  The kernel
+    resource remarks list spills per function: every pipelined dist_tiles /
+    batch_tile_k / dist_batch instance must report none."""
+    name, bad = None, []
+    for line in remarks.splitlines():
+        if "Function Name:" in line:
+            name = line.split("Function Name:")[1].split()[0]
+        elif name and ("SGPRs Spill:" in line or "VGPRs Spill:" in line):
+            pipelined = ("dist_tiles" in name or "batch_tile_k" in name or
+                         "dist_batch" in name) and "ELi2EE" in name
+            if pipelined and int(line.rsplit(":", 1)[1].split()[0]) != 0:
+                bad.append(name + ": " + line.split("remark:")[-1].strip())
+    if bad:
+        raise RuntimeError("pipelined tile kernels spill registers:\n  " + "\n  ".join(bad))
+
+
 def build_lib():
     objs = []
     for src in HIP_SOURCES:
         obj = os.path.join(HERE, "_build", os.path.splitext(src)[0] + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
-        _run([HIPCC, "-x", "hip"] + HIP_FLAGS + ["-I", INCLUDE, "-c",
-                                                  os.path.join(CSRC, src), "-o", obj])
+        cmd = [HIPCC, "-x", "hip"] + HIP_FLAGS + ["-I", INCLUDE, "-c",
+                                                  os.path.join(CSRC, src), "-o", obj]
+        if src == "dist_kernels.hip":
+            cmd.append("-Rpass-analysis=kernel-resource-usage")
+            print("+", " ".join(cmd), flush=True)
+            r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                print(r.stdout)
+                raise RuntimeError("build step failed: " + " ".join(cmd))
+            _check_pipelined_spills(r.stdout)
+        else:
+            _run(cmd)
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(HERE, "_build", os.path.splitext(src)[0] + ".o")

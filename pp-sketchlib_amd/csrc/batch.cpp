@@ -45,7 +45,7 @@ struct ppsk_batch {
 namespace {
 
 // tile shapes the persistent kernel is built for, tried by the autotuner
-const int kBatchVariants[] = {2, 6};
+const int kBatchVariants[] = {2, 4};
 
 std::mutex g_tune_mu;
 std::vector<std::pair<uint64_t, int>> g_tuned;  // job-shape signature -> tile shape
@@ -88,7 +88,8 @@ int build_plan(ppsk_batch* b, int variant) {
   std::vector<size_t> gs;
   for (int j : order) {
     const ppsk_job& jb = b->jobs[j];
-    auto tiles = make_tiles(jb.query->n, jb.ref->n, tr, tc, false, 0, jb.query->n, tile_group(), &gs);
+    const bool self = jb.query == jb.ref;
+    auto tiles = make_tiles(jb.query->n, jb.ref->n, tr, tc, self, 0, jb.query->n, tile_group(), &gs);
     const size_t n = tiles.size();
     for (int qi = 0; qi < ppsk::BATCH_QUEUES; ++qi) {
       const size_t a = n * qi / ppsk::BATCH_QUEUES, e = n * (qi + 1) / ppsk::BATCH_QUEUES;
@@ -208,12 +209,13 @@ int ppsk_batch_create(int device_id, const ppsk_job* jobs, int32_t n_jobs, ppsk_
     if ((rc = upload_table(c, jb.query, rnd, &b->tables[j]))) break;
     ppsk::DistParams& p = b->params[j];
     fill_common(p, jb.query, jb.ref, rnd, jb.tolerance);
-    p.self_mode = 0;
+    const bool self = jb.query == jb.ref;  // self job: condensed upper triangle
+    p.self_mode = self ? 1 : 0;
     p.full_square = 0;
     p.out_kind = jb.out_kind;
     p.out = jb.out;
     p.mc_table = b->tables[j];
-    p.mc_ref_is_col = 1;  // query mode: random_match(r1 = ref, q)
+    p.mc_ref_is_col = self ? 0 : 1;  // query mode: random_match(r1 = ref, q)
     p.fail_count = b->d_fail + 2 * j;
     p.fail_first = b->d_fail + 2 * j + 1;
     b->max_k = std::max(b->max_k, jb.ref->n_k);
@@ -240,7 +242,7 @@ int ppsk_batch_run(ppsk_batch* b, int32_t variant, uint64_t* n_fail, int32_t* us
     bool ok = false;
     for (int v : kBatchVariants) ok |= (v == variant);
     if (!ok) return fail(PPSK_ERR_ARG, "tile shape " + std::to_string(variant) +
-                                           " has no batch kernel (2 or 6, or 0 to autotune)");
+                                           " has no batch kernel (2 or 4, or 0 to autotune)");
   }
   DeviceCtx* c = nullptr;
   if (int rc = get_ctx(b->device, &c)) return rc;

@@ -159,11 +159,14 @@ __device__ __forceinline__ uint32_t lds_addr(const uint4* p) {
 // tell the two stage buffers apart and would wait for the copy of the next
 // stage before reading the current one); completion is awaited explicitly
 // with vmcnt before the stage barrier.
-__device__ __forceinline__ void dma16(const uint32_t* g, uint32_t lds_byte_addr) {
-  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
+// The LDS address is base + OFF with OFF an immediate, so a stage needs one
+// SGPR per buffer rather than one per copy.
+template <int OFF>
+__device__ __forceinline__ void dma16(const uint32_t* g, uint32_t lds_base) {
+  asm volatile("s_add_u32 m0, %0, %2\n\tglobal_load_lds_dwordx4 %1, off"
                :
-               : "s"(lds_byte_addr), "v"(g)
-               : "memory", "m0");
+               : "s"(lds_base), "v"(g), "i"(OFF)
+               : "memory", "m0", "scc");
 }
 
 // Workgroup barrier without the release/acquire fences of __syncthreads():
@@ -175,15 +178,15 @@ __device__ __forceinline__ void stage_barrier() { asm volatile("s_barrier" ::: "
 // Column operands of one stage: words [w0, w0 + WS) of the tile's TC columns,
 // which may span two stripe blocks (TC = 256) or half of one (TC = 64).
 // Columns past the last stripe block read the last block (masked later).
-template <int RQ, int RR>
-__device__ __forceinline__ void issue_stage(const uint32_t* __restrict__ col_kk, int64_t col0,
-                                            int nb_cols, int s64, int w0, uint4* buf, int wave,
-                                            int lane) {
+// Round I of wave w copies uint4 [I * 512 + w * 64, +64) of the stage.
+template <int RQ, int RR, int I>
+__device__ __forceinline__ void issue_round(const uint32_t* __restrict__ col_kk, int64_t col0,
+                                            int nb_cols, int s64, int w0, uint32_t wbase,
+                                            int wave, int lane) {
   using Cfg = TileCfg<RQ, RR>;
   constexpr int TC = Cfg::TC;
-#pragma unroll
-  for (int i = 0; i < Cfg::ROUNDS; ++i) {
-    const int x0 = i * (WAVES * 64) + wave * 64;  // wave-uniform
+  if constexpr (I < Cfg::ROUNDS) {
+    const int x0 = I * (WAVES * 64) + wave * 64;  // wave-uniform
     if (x0 < Cfg::STAGE_U4) {
       const int x = x0 + lane;
       const int row = x / (TC / 4);  // ws * HL + hl
@@ -192,9 +195,18 @@ __device__ __forceinline__ void issue_stage(const uint32_t* __restrict__ col_kk,
       blk = blk < nb_cols ? blk : nb_cols - 1;
       const uint32_t* g =
           col_kk + ((int64_t)blk * s64 + w0) * (HL * BLK) + (int64_t)row * BLK + (col % BLK);
-      dma16(g, lds_addr(buf + x0));
+      dma16<I * WAVES * 64 * 16>(g, wbase);
     }
+    issue_round<RQ, RR, I + 1>(col_kk, col0, nb_cols, s64, w0, wbase, wave, lane);
   }
+}
+
+template <int RQ, int RR>
+__device__ __forceinline__ void issue_stage(const uint32_t* __restrict__ col_kk, int64_t col0,
+                                            int nb_cols, int s64, int w0, uint4* buf, int wave,
+                                            int lane) {
+  const uint32_t wbase = lds_addr(buf) + (uint32_t)(wave * 64 * 16);
+  issue_round<RQ, RR, 0>(col_kk, col0, nb_cols, s64, w0, wbase, wave, lane);
 }
 
 // Loads still outstanding for a wave after it issued one more stage.
@@ -276,6 +288,10 @@ template <>
 struct ATuple<4> {
   typedef uint32_t T __attribute__((ext_vector_type(4)));
 };
+template <>
+struct ATuple<2> {
+  typedef uint32_t T __attribute__((ext_vector_type(2)));
+};
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 template <int RQ, int RR, int PB>
@@ -291,9 +307,11 @@ __device__ __forceinline__ void a_load(const CU32* aw, typename ATuple<RQ>::T& x
     asm volatile("s_load_dwordx16 %0, %1, %2" : "=&s"(x) : "s"(aw), "i"(OFF));
   } else if constexpr (RQ == 8) {
     asm volatile("s_load_dwordx8 %0, %1, %2" : "=&s"(x) : "s"(aw), "i"(OFF));
-  } else {
-    static_assert(RQ == 4, "row operands come in SGPR tuples of 4, 8 or 16");
+  } else if constexpr (RQ == 4) {
     asm volatile("s_load_dwordx4 %0, %1, %2" : "=&s"(x) : "s"(aw), "i"(OFF));
+  } else {
+    static_assert(RQ == 2, "row operands come in SGPR tuples of 2, 4, 8 or 16");
+    asm volatile("s_load_dwordx2 %0, %1, %2" : "=&s"(x) : "s"(aw), "i"(OFF));
   }
 }
 
@@ -744,19 +762,27 @@ __global__ void batch_reset_kernel(uint32_t* queue_next, unsigned long long* fai
 bool variant_valid(int v) { return v >= 1 && v <= N_VARIANTS; }
 
 int variant_tile_rows(int v) {
-  if (v == 6) return 32;
-  return (v == 2 || v == 5) ? 64 : 128;
+  switch (v) {
+    case 1:
+    case 3:
+      return 128;
+    case 4:
+      return 32;
+    case 5:
+      return 16;
+    default:
+      return 64;
+  }
 }
 
 int variant_tile_cols(int v) {
   switch (v) {
     case 1:
       return 64;
-    case 2:
-    case 6:
-      return 256;
-    default:
+    case 3:
       return 128;
+    default:
+      return 256;
   }
 }
 
@@ -771,13 +797,10 @@ hipError_t launch_dists(const DistParams& p, int variant, int64_t n_items, hipSt
       dist_tiles<16, 2, 0><<<g, WAVES * 64, 0, stream>>>(p);
       break;
     case 4:
-      dist_tiles<16, 2, 1><<<g, WAVES * 64, 0, stream>>>(p);
+      dist_tiles<4, 4, 2><<<g, WAVES * 64, 0, stream>>>(p);
       break;
     case 5:
-      dist_tiles<8, 2, 2><<<g, WAVES * 64, 0, stream>>>(p);
-      break;
-    case 6:
-      dist_tiles<4, 4, 2><<<g, WAVES * 64, 0, stream>>>(p);
+      dist_tiles<2, 4, 2><<<g, WAVES * 64, 0, stream>>>(p);
       break;
     default:
       dist_tiles<8, 4, 2><<<g, WAVES * 64, 0, stream>>>(p);
@@ -790,9 +813,7 @@ hipError_t launch_dists(const DistParams& p, int variant, int64_t n_items, hipSt
 template <typename F>
 static hipError_t with_batch_kernel(int variant, F f) {
   switch (variant) {
-    case 5:
-      return f(dist_batch<8, 2, 2>);
-    case 6:
+    case 4:
       return f(dist_batch<4, 4, 2>);
     case 2:
       return f(dist_batch<8, 4, 2>);

@@ -156,26 +156,19 @@ int check_random(const ppsk_set* rows, const ppsk_set* cols, const RandomIn& rnd
   return PPSK_OK;
 }
 
-// Row tiles per scheduling group (make_tiles); PPSK_TILE_GROUP overrides.
+// Row tiles per scheduling group (make_tiles); PPSK_TILE_GROUP overrides
+// (read on every call, so a process can compare settings).
 int tile_group() {
-  static int g = -1;
-  if (g < 0) {
-    const char* e = std::getenv("PPSK_TILE_GROUP");
-    g = e ? std::atoi(e) : 8;
-    if (g < 1) g = 1;
-  }
-  return g;
+  const char* e = std::getenv("PPSK_TILE_GROUP");
+  const int g = e ? std::atoi(e) : 8;
+  return g < 1 ? 1 : g;
 }
 
 // Tile kernel configuration: PPSK_TILE_VARIANT overrides the default.
 int default_variant() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = std::getenv("PPSK_TILE_VARIANT");
-    v = e ? std::atoi(e) : ppsk::DEFAULT_VARIANT;
-    if (!ppsk::variant_valid(v)) v = ppsk::DEFAULT_VARIANT;
-  }
-  return v;
+  const char* e = std::getenv("PPSK_TILE_VARIANT");
+  const int v = e ? std::atoi(e) : ppsk::DEFAULT_VARIANT;
+  return ppsk::variant_valid(v) ? v : ppsk::DEFAULT_VARIANT;
 }
 
 

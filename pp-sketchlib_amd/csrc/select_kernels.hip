@@ -105,10 +105,17 @@ __global__ __launch_bounds__(64) void knn_select_kernel(Rows rows, int64_t r0, i
   const int lane = threadIdx.x;
   int len = 0;        // wave-uniform heap size
   float top = 0.0f;   // heap top once len == kNN
+  // the next chunk's distances are loaded before the current chunk is
+  // scanned, so the row streams at memory throughput rather than latency
+  auto load = [&](int64_t c) {
+    const int64_t jj = c + lane;
+    return (jj < n && !(excl && jj == gi)) ? rows(gi, jj) : INFINITY;
+  };
+  float d_next = load(0);
   for (int64_t c0 = 0; c0 < n; c0 += 64) {
     const int64_t j = c0 + lane;
-    float d = INFINITY;
-    if (j < n && !(excl && j == gi)) d = rows(gi, j);
+    const float d = d_next;
+    d_next = load(c0 + 64);
     const bool cand = (j < n) && (len < kNN || d < top);
     unsigned long long mask = __ballot(cand);
     while (mask) {

@@ -120,7 +120,7 @@ struct ppsk_server {
   int32_t kNN = 0, knn_col = 0, thr_col = 0;
   float cutoff = 0;
   int32_t max_rows = 0;
-  int variant = 2;
+  int variant = 0;  // tile shape; 0: by step size (step_variant)
   size_t per_sample = 0;
   size_t scratch_budget = 0;
   Arena arena;
@@ -141,6 +141,15 @@ struct ppsk_server {
 
 namespace {
 
+// Tile shape of a step: the shortest row tile that holds the step's rows
+// (a step of a few queries would otherwise compute mostly padding rows).
+int step_variant(const ppsk_server* s, int64_t rows) {
+  if (s->variant) return s->variant;
+  if (rows <= 16) return 5;
+  if (rows <= 32) return 4;
+  return 2;
+}
+
 struct StepLayout {
   size_t staging, stripes, jv, band, tiles, scratch, tile_done, fail, fail_rows, total;
   int64_t nbq, n_tiles, chunk_tiles;
@@ -149,8 +158,9 @@ struct StepLayout {
 StepLayout layout(const ppsk_server* s, int64_t rows) {
   const ppsk_set* r = s->ref;
   StepLayout L{};
-  const int tr = ppsk::variant_tile_rows(s->variant);
-  const int tc = ppsk::variant_tile_cols(s->variant);
+  const int v = step_variant(s, rows);
+  const int tr = ppsk::variant_tile_rows(v);
+  const int tc = ppsk::variant_tile_cols(v);
   L.nbq = (rows + ppsk::BLK - 1) / ppsk::BLK;
   L.n_tiles = ((rows + tr - 1) / tr) * ((r->n + tc - 1) / tc);
   const size_t per_tile = (size_t)r->n_k * tr * tc * sizeof(uint16_t);
@@ -266,8 +276,9 @@ int run_step(ppsk_server* s, Worker& w, const std::vector<ppsk_request*>& batch,
   p.fail_count = fail;
   p.fail_first = fail + 1;
   p.fail_rows = fail_rows;
-  const int tr = ppsk::variant_tile_rows(s->variant);
-  const int tc = ppsk::variant_tile_cols(s->variant);
+  const int variant = step_variant(s, rows);
+  const int tr = ppsk::variant_tile_rows(variant);
+  const int tc = ppsk::variant_tile_cols(variant);
   std::vector<size_t> gs;
   const auto tiles = make_tiles(rows, ref->n, tr, tc, false, 0, rows, tile_group(), &gs);
   STEP_TRY(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(ppsk::Tile),
@@ -278,7 +289,7 @@ int run_step(ppsk_server* s, Worker& w, const std::vector<ppsk_request*>& batch,
     p.tiles = d_tiles + a;
     p.n_tiles = (int64_t)(b - a);
     p.item_base = 0;
-    STEP_TRY(ppsk::launch_dists(p, s->variant, (int64_t)(b - a) * n_k, w.st));
+    STEP_TRY(ppsk::launch_dists(p, variant, (int64_t)(b - a) * n_k, w.st));
   }
   STEP_TRY(hipEventRecord(w.e1, w.st));
 
@@ -456,7 +467,7 @@ int ppsk_server_create(ppsk_set* ref, int32_t random_mode, double tolerance, int
   s->cutoff = cutoff;
   s->thr_col = thr_col;
   s->max_rows = max_rows;
-  s->variant = default_variant();
+  s->variant = std::getenv("PPSK_TILE_VARIANT") ? default_variant() : 0;
   s->per_sample = (size_t)ref->n_k * ref->s64 * ppsk::BBITS * sizeof(uint64_t);
   s->scratch_budget = (size_t)256 << 20;
   if (arena_bytes == 0) {

@@ -237,7 +237,8 @@ class Batch:
             mode, ncl, tab = _random_args(j.get("random_mode", RANDOM_NONE), j.get("table"))
             tol = j.get("tol")
             tol = default_tolerance(ref.s64) if tol is None else tol
-            rows = query.n * ref.n
+            self_job = query is ref
+            rows = ref.n * (ref.n - 1) // 2 if self_job else query.n * ref.n
             n_k = len(ref.klist)
             if kind == 0:
                 shape, dt = (rows, 2), np.float32
@@ -270,7 +271,7 @@ class Batch:
 
     def tuning(self):
         v = _i32()
-        ms = np.zeros(8, np.float64)
+        ms = np.zeros(6, np.float64)
         _check(lib().ppsk_batch_tuning(self._h, _c.byref(v), _ptr(ms)))
         return v.value, ms
 

@@ -50,8 +50,10 @@ def _mode(name):
 def dbs():
     out = []
     for ci, (kl, s64, nr, nq) in enumerate(CLASSES):
-        raw, le, _ = H.synth(nr, kl, s64, seed=100 + ci)
-        qraw, qle, _ = H.synth(nq, kl, s64, seed=200 + ci)
+        # references and queries from one tree, so every pair is related
+        allr, alle, _ = H.synth(nr + nq, kl, s64, seed=100 + ci)
+        raw, le = allr[:nr].copy(), alle[:nr].copy()
+        qraw, qle = allr[nr:].copy(), alle[nr:].copy()
         can_len = max(kl) < 33
         rcl = (np.arange(nr) * 7 % 3).astype(np.uint16)
         qcl = (np.arange(nq) % 3).astype(np.uint16)
@@ -95,9 +97,9 @@ def batch_run(dbs):
 def test_batch_autotuned_equals_single_calls(batch_run):
     b, views = batch_run
     nf, used, ms = b.run()  # first run: autotuner times every shape
-    assert used in (2, 6) and ms > 0
+    assert used in (2, 4) and ms > 0
     chosen, per = b.tuning()
-    assert chosen == used and per[2] > 0 and per[6] > 0
+    assert chosen == used and per[2] > 0 and per[4] > 0
     assert (nf == 0).all()
     for x, spec in enumerate(JOBS):
         d, ref, qry = views[x]
@@ -108,7 +110,7 @@ def test_batch_autotuned_equals_single_calls(batch_run):
         assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), f"job {x} {spec}"
 
 
-@pytest.mark.parametrize("variant", [2, 6])
+@pytest.mark.parametrize("variant", [2, 4])
 def test_batch_each_shape_and_graph_replays(batch_run, variant):
     # three replays of the captured graph, with other device work between
     # them, all equal to the single calls

@@ -185,3 +185,24 @@ def test_knn_asymmetric_table_band_path():
     assert nf == rnf == 0
     assert np.array_equal(iv, ri) and np.array_equal(jv, rj)
     assert max_abs_diff(dv, rd) <= CORE_ACC_TOL
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+def test_every_tile_shape(monkeypatch, variant):
+    # every compiled tile configuration, self and query mode, against the
+    # oracle (sample counts not a multiple of any tile edge)
+    monkeypatch.setenv("PPSK_TILE_VARIANT", str(variant))
+    raw, le, _ = synth(300)
+    with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
+        c, _, _ = H.dense(s, out="counts")
+        d, nf, _ = H.dense(s, random_mode=H.RANDOM_EQUAL_BASE)
+        with H.SketchSet(raw[:45].copy(), KLIST8, 156, lengths=le[:45].copy()) as q:
+            qd, _, _ = H.dense(s, q, random_mode=H.RANDOM_EQUAL_BASE)
+    ref, ref_c, _, _ = O.query_db(raw, KLIST8, 156, 14, le, random=O.Random(1, True),
+                                  want_counts=True)
+    qref, _, _, _ = O.query_db(raw, KLIST8, 156, 14, le, query=raw[:45], query_len=le[:45],
+                               random=O.Random(1, True))
+    assert nf == 0
+    assert np.array_equal(c, ref_c)
+    assert np.abs(d - ref).max() <= CORE_ACC_TOL
+    assert np.abs(qd - qref).max() <= CORE_ACC_TOL
