@@ -171,7 +171,7 @@ def main():
         "unit": "TOP/s",
         "frac": round(achieved / PEAK_TOPS, 4),
         "traffic": traffic_from_profile(),
-        "kernel": "dist_tiles",
+        "kernel": "dist_batch",
         "kernel_ms": round(per_launch_ms, 3),
         "ops_per_pair": OPS_PER_PAIR,
         "peak_measured": None,

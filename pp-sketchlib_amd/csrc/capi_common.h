@@ -66,6 +66,13 @@ struct DeviceCtx {
   size_t out_bytes = 0;
   hipStream_t copy_stream = nullptr;  // device -> host copies of finished rows
   std::vector<hipEvent_t> events;     // per-chunk timing / copy dependencies
+  // persistent launches (dist_batch over the tiles of one call)
+  void* pscratch = nullptr;           // per-workgroup count blocks
+  size_t pscratch_bytes = 0;
+  void* pitems = nullptr;             // BatchItem list
+  size_t pitems_bytes = 0;
+  ppsk::DistParams* pparams = nullptr;  // the call's parameters in device memory
+  uint32_t* pqueue = nullptr;         // queue counters
   std::mutex mu;
 };
 

@@ -153,19 +153,19 @@ __device__ __forceinline__ uint32_t lds_addr(const uint4* p) {
   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
 }
 
-// One 16-byte-per-lane global -> LDS copy: lane L writes LDS bytes
-// [m0 + 16 L, +16).  Written as inline asm so that the compiler's wait-count
-// pass does not treat it as a pending write to every LDS buffer (it cannot
-// tell the two stage buffers apart and would wait for the copy of the next
-// stage before reading the current one); completion is awaited explicitly
-// with vmcnt before the stage barrier.
-// The LDS address is base + OFF with OFF an immediate, so a stage needs one
-// SGPR per buffer rather than one per copy.
+// One 16-byte-per-lane global -> LDS copy: lane L reads saddr + voff (bytes)
+// and writes LDS bytes [lds_base + OFF + 16 L, +16).  Written as inline asm
+// so that the compiler's wait-count pass does not treat it as a pending write
+// to every LDS buffer (it cannot tell the two stage buffers apart and would
+// wait for the copy of the next stage before reading the current one);
+// completion is awaited explicitly with vmcnt before the stage barrier.  The
+// LDS address is base + OFF with OFF an immediate, so a stage needs one SGPR
+// per buffer rather than one per copy.
 template <int OFF>
-__device__ __forceinline__ void dma16(const uint32_t* g, uint32_t lds_base) {
-  asm volatile("s_add_u32 m0, %0, %2\n\tglobal_load_lds_dwordx4 %1, off"
+__device__ __forceinline__ void dma16(const uint32_t* saddr, uint32_t voff, uint32_t lds_base) {
+  asm volatile("s_add_u32 m0, %0, %3\n\tglobal_load_lds_dwordx4 %1, %2"
                :
-               : "s"(lds_base), "v"(g), "i"(OFF)
+               : "s"(lds_base), "v"(voff), "s"(saddr), "i"(OFF)
                : "memory", "m0", "scc");
 }
 
@@ -179,47 +179,56 @@ __device__ __forceinline__ void stage_barrier() { asm volatile("s_barrier" ::: "
 // which may span two stripe blocks (TC = 256) or half of one (TC = 64).
 // Columns past the last stripe block read the last block (masked later).
 // Round I of wave w copies uint4 [I * 512 + w * 64, +64) of the stage.
-template <int RQ, int RR, int I>
-__device__ __forceinline__ void issue_round(const uint32_t* __restrict__ col_kk, int64_t col0,
-                                            int nb_cols, int s64, int w0, uint32_t wbase,
-                                            int wave, int lane) {
+//
+// Addressing: a lane's source is (tile column base) + (w0 * plane slab) +
+// (its byte offset within the tile's <= 2 stripe blocks).  The last term does
+// not depend on the word, so it is computed once per tile (ColOffsets, one
+// VGPR per round, < 2^32 since it spans at most two blocks of one word group)
+// and every stage only moves the scalar base.
+template <int RQ, int RR>
+struct ColOffsets {
+  uint32_t v[TileCfg<RQ, RR>::ROUNDS];
+};
+
+template <int RQ, int RR>
+__device__ __forceinline__ ColOffsets<RQ, RR> col_offsets(int64_t col0, int nb_cols, int s64,
+                                                          int wave, int lane) {
   using Cfg = TileCfg<RQ, RR>;
   constexpr int TC = Cfg::TC;
-  if constexpr (I < Cfg::ROUNDS) {
-    const int x0 = I * (WAVES * 64) + wave * 64;  // wave-uniform
-    if (x0 < Cfg::STAGE_U4) {
-      const int x = x0 + lane;
-      const int row = x / (TC / 4);  // ws * HL + hl
-      const int64_t col = col0 + (x % (TC / 4)) * 4;
-      int blk = (int)(col / BLK);
-      blk = blk < nb_cols ? blk : nb_cols - 1;
-      const uint32_t* g =
-          col_kk + ((int64_t)blk * s64 + w0) * (HL * BLK) + (int64_t)row * BLK + (col % BLK);
-      dma16<I * WAVES * 64 * 16>(g, wbase);
-    }
-    issue_round<RQ, RR, I + 1>(col_kk, col0, nb_cols, s64, w0, wbase, wave, lane);
+  ColOffsets<RQ, RR> o;
+  const int blk0 = (int)(col0 / BLK);
+#pragma unroll
+  for (int i = 0; i < Cfg::ROUNDS; ++i) {
+    const int x = i * (WAVES * 64) + wave * 64 + lane;
+    const int row = x / (TC / 4);  // ws * HL + hl
+    const int64_t col = col0 + (x % (TC / 4)) * 4;
+    int blk = (int)(col / BLK);
+    blk = blk < nb_cols ? blk : nb_cols - 1;
+    const uint32_t e = (uint32_t)(blk - blk0) * (uint32_t)s64 * (HL * BLK) +
+                       (uint32_t)row * BLK + (uint32_t)(col % BLK);
+    o.v[i] = e * 4u;
   }
+  return o;
 }
 
-template <int RQ, int RR>
-__device__ __forceinline__ void issue_stage(const uint32_t* __restrict__ col_kk, int64_t col0,
-                                            int nb_cols, int s64, int w0, uint4* buf, int wave,
-                                            int lane) {
-  const uint32_t wbase = lds_addr(buf) + (uint32_t)(wave * 64 * 16);
-  issue_round<RQ, RR, 0>(col_kk, col0, nb_cols, s64, w0, wbase, wave, lane);
-}
-
-// Loads still outstanding for a wave after it issued one more stage.
-template <int RQ, int RR>
-__device__ __forceinline__ void wait_stage_but_one(int wave) {
+template <int RQ, int RR, int I>
+__device__ __forceinline__ void issue_round(const uint32_t* sbase, const ColOffsets<RQ, RR>& off,
+                                            uint32_t wbase, int wave) {
   using Cfg = TileCfg<RQ, RR>;
-  constexpr int FULL = Cfg::STAGE_U4 / (WAVES * 64);  // rounds every wave issues
-  constexpr int REM_WAVES = (Cfg::STAGE_U4 % (WAVES * 64)) / 64;
-  if (wave < REM_WAVES) {
-    wait_vmcnt<FULL + 1>();
-  } else {
-    wait_vmcnt<FULL>();
+  if constexpr (I < Cfg::ROUNDS) {
+    if (I * (WAVES * 64) + wave * 64 < Cfg::STAGE_U4) {  // wave-uniform
+      dma16<I * WAVES * 64 * 16>(sbase, off.v[I], wbase);
+    }
+    issue_round<RQ, RR, I + 1>(sbase, off, wbase, wave);
   }
+}
+
+// sbase: the tile's first column block, word w0 (wave-uniform).
+template <int RQ, int RR>
+__device__ __forceinline__ void issue_stage(const uint32_t* sbase, const ColOffsets<RQ, RR>& off,
+                                            uint4* buf, int wave) {
+  const uint32_t wbase = lds_addr(buf) + (uint32_t)(wave * 64 * 16);
+  issue_round<RQ, RR, 0>(sbase, off, wbase, wave);
 }
 
 // ---------------------------------------------------------------------------
@@ -578,9 +587,15 @@ __device__ __forceinline__ void tile_match(CParams* pp, Tile tile, int kk, uint4
     for (int r = 0; r < RR; ++r) cnt[q][r] = 0;
 
   const int n_stages = (s64 + WS - 1) / WS;
-  issue_stage<RQ, RR>(col_kk, col0, nb_cols, s64, 0, sB0, wave, lane);
-  // stages alternate between the two buffers; each pass of this loop runs an
-  // even stage from sB0 and the following odd stage from sB1
+  const ColOffsets<RQ, RR> off = col_offsets<RQ, RR>(col0, nb_cols, s64, wave, lane);
+  const uint32_t* cbase = col_kk + (int64_t)(col0 / BLK) * s64 * (HL * BLK);
+  issue_stage<RQ, RR>(cbase, off, sB0, wave);
+  // One barrier per stage: after it, stage s is in LDS for every wave and
+  // every wave has finished matching stage s - 1, whose buffer the copy of
+  // stage s + 1 (issued right after the barrier) then overwrites while stage
+  // s is matched.  Stages alternate between the two buffers; each pass of
+  // this loop runs an even stage from sB0 and the following odd stage from
+  // sB1.
   for (int st = 0; st < n_stages; st += 2) {
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
@@ -589,13 +604,11 @@ __device__ __forceinline__ void tile_match(CParams* pp, Tile tile, int kk, uint4
         uint4* cur = half ? sB1 : sB0;
         uint4* nxt = half ? sB0 : sB1;
         const int w0 = s * WS;
-        if (s + 1 < n_stages) {
-          issue_stage<RQ, RR>(col_kk, col0, nb_cols, s64, w0 + WS, nxt, wave, lane);
-          wait_stage_but_one<RQ, RR>(wave);
-        } else {
-          wait_vmcnt<0>();
-        }
+        wait_vmcnt<0>();
         stage_barrier();
+        if (s + 1 < n_stages) {
+          issue_stage<RQ, RR>(cbase + (int64_t)(w0 + WS) * (HL * BLK), off, nxt, wave);
+        }
         const int nw = (s64 - w0) < WS ? (s64 - w0) : WS;
         const uint32_t* sB = reinterpret_cast<const uint32_t*>(cur);
         if constexpr (PB == 0) {
@@ -603,10 +616,11 @@ __device__ __forceinline__ void tile_match(CParams* pp, Tile tile, int kk, uint4
         } else {
           match_stage_pipe<RQ, RR, PB>(arow, w0, nw, cur, lane, cnt);
         }
-        stage_barrier();
       }
     }
   }
+  // every wave is done with both buffers before they are reused
+  stage_barrier();
 }
 
 // Counts of this wave / lane into a tile count block [n_k][rows][cols].
@@ -753,7 +767,7 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_batch(BatchArgs a) {
 __global__ void batch_reset_kernel(uint32_t* queue_next, unsigned long long* fail, int n_jobs) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x < BATCH_QUEUES) queue_next[x] = 0u;
-  if (x < n_jobs) {
+  if (fail && x < n_jobs) {
     fail[2 * x] = 0ull;
     fail[2 * x + 1] = ~0ull;
   }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -197,8 +198,8 @@ int64_t out_row_start(bool self, int64_t r, int64_t n_rows, int64_t n_cols) {
 // the per-tile count scratch; when host_out is set, the finished output rows
 // of each launch are copied to host_out on the copy stream while the
 // following launches compute.  Returns after both streams have drained.
-int run_dense(DeviceCtx* c, ppsk::DistParams p, int variant, bool upper, int64_t row_begin,
-              int64_t row_end, bool self, size_t elem, void* host_out, Timing* tm) {
+int run_dense_tiles(DeviceCtx* c, ppsk::DistParams p, int variant, bool upper, int64_t row_begin,
+                    int64_t row_end, bool self, size_t elem, void* host_out, Timing* tm) {
   const int tr = ppsk::variant_tile_rows(variant);
   const int tc = ppsk::variant_tile_cols(variant);
   const size_t per_tile = (size_t)p.n_k * tr * tc * sizeof(uint16_t);
@@ -281,6 +282,127 @@ int run_dense(DeviceCtx* c, ppsk::DistParams p, int variant, bool upper, int64_t
     tm->ms += ms;
   }
   return PPSK_OK;
+}
+
+// Copies the finished dense rows of tiles [a, b) to the host on the copy
+// stream once `done` has been reached on the launch stream.
+int copy_rows(DeviceCtx* c, const std::vector<ppsk::Tile>& tiles, size_t a, size_t b, int tr,
+              const ppsk::DistParams& p, bool self, size_t elem, void* host_out, hipEvent_t done) {
+  int32_t bmin = tiles[a].bi, bmax = tiles[a].bi;
+  for (size_t x = a; x < b; ++x) {
+    bmin = std::min(bmin, tiles[x].bi);
+    bmax = std::max(bmax, tiles[x].bi);
+  }
+  const int64_t o0 = out_row_start(self, (int64_t)bmin * tr, p.n_rows, p.n_cols);
+  const int64_t o1 = out_row_start(self, (int64_t)(bmax + 1) * tr, p.n_rows, p.n_cols);
+  if (o1 > o0) {
+    const char* dev = (const char*)p.out;
+    HIP_TRY(hipStreamWaitEvent(c->copy_stream, done, 0));
+    HIP_TRY(hipMemcpyAsync((char*)host_out + o0 * elem, dev + o0 * elem, (o1 - o0) * elem,
+                           hipMemcpyDeviceToHost, c->copy_stream));
+  }
+  return PPSK_OK;
+}
+
+// The same computation as run_dense_tiles with the persistent kernel: whole
+// tiles (every k-mer length, then the epilogue) are work items of one launch
+// over every CU, counts kept in a per-workgroup block.  For host output the
+// rows are cut into about eight launches (whole groups of row tiles) so that
+// each launch's rows are copied out while the next computes.
+int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upper,
+                         int64_t row_begin, int64_t row_end, bool self, size_t elem,
+                         void* host_out, Timing* tm) {
+  const int tr = ppsk::variant_tile_rows(variant);
+  const int tc = ppsk::variant_tile_cols(variant);
+  // PPSK_CHUNK_BYTES (host output): one row tile per group and per launch,
+  // the finest streaming granularity
+  const bool finest = host_out && std::getenv("PPSK_CHUNK_BYTES");
+  std::vector<size_t> gs;
+  const std::vector<ppsk::Tile> tiles = make_tiles(p.n_rows, p.n_cols, tr, tc, upper, row_begin,
+                                                   row_end, finest ? 1 : tile_group(), &gs);
+  if (tiles.empty()) return PPSK_OK;
+  int grid = 0;
+  HIP_TRY(ppsk::batch_grid(variant, &grid));
+  // launches: runs of whole groups of about tiles / 8 (host output) or one
+  std::vector<std::pair<size_t, size_t>> chunks;
+  size_t target = host_out ? std::max<size_t>((size_t)grid * 4, tiles.size() / 8) : tiles.size();
+  if (finest) target = 1;
+  for (size_t g = 0; g + 1 < gs.size();) {
+    size_t h = g + 1;
+    while (h + 1 < gs.size() && gs[h] - gs[g] < target) ++h;
+    chunks.emplace_back(gs[g], gs[h]);
+    g = h;
+  }
+  // items: chunk by chunk, each chunk's tiles cut into one contiguous run per
+  // queue (so each XCD walks a compact block)
+  std::vector<ppsk::BatchItem> items;
+  items.reserve(tiles.size());
+  std::vector<std::array<int64_t, ppsk::BATCH_QUEUES + 1>> qb(chunks.size());
+  for (size_t ci = 0; ci < chunks.size(); ++ci) {
+    const size_t a = chunks[ci].first, n = chunks[ci].second - a;
+    for (int q = 0; q < ppsk::BATCH_QUEUES; ++q) {
+      qb[ci][q] = (int64_t)items.size();
+      for (size_t x = a + n * q / ppsk::BATCH_QUEUES; x < a + n * (q + 1) / ppsk::BATCH_QUEUES; ++x) {
+        items.push_back({0, tiles[x].bi, tiles[x].ct, 0});
+      }
+    }
+    qb[ci][ppsk::BATCH_QUEUES] = (int64_t)items.size();
+  }
+  const size_t scratch = (size_t)grid * p.n_k * tr * tc * sizeof(uint16_t);
+  if (int rc = grow(&c->pscratch, &c->pscratch_bytes, scratch)) return rc;
+  if (int rc = grow(&c->pitems, &c->pitems_bytes, items.size() * sizeof(ppsk::BatchItem))) return rc;
+  if (!c->pparams) HIP_TRY(hipMalloc(&c->pparams, sizeof(ppsk::DistParams)));
+  if (!c->pqueue) HIP_TRY(hipMalloc(&c->pqueue, ppsk::BATCH_QUEUES * sizeof(uint32_t)));
+  p.out_row_offset = 0;
+  HIP_TRY(hipMemcpyAsync(c->pitems, items.data(), items.size() * sizeof(ppsk::BatchItem),
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->pparams, &p, sizeof(p), hipMemcpyHostToDevice, c->stream));
+  ppsk::BatchArgs args;
+  std::memset(&args, 0, sizeof(args));
+  args.jobs = c->pparams;
+  args.items = (const ppsk::BatchItem*)c->pitems;
+  args.queue_next = c->pqueue;
+  args.scratch = (uint16_t*)c->pscratch;
+  args.max_k = p.n_k;
+  for (size_t ci = 0; ci < chunks.size(); ++ci) {
+    for (int q = 0; q <= ppsk::BATCH_QUEUES; ++q) args.queue_begin[q] = qb[ci][q];
+    hipEvent_t e0 = pool_event(c, 2 * ci), e1 = pool_event(c, 2 * ci + 1);
+    if (!e0 || !e1) return fail(PPSK_ERR_HIP, "event creation failed");
+    HIP_TRY(ppsk::launch_batch_reset(c->pqueue, nullptr, 0, c->stream));
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    HIP_TRY(ppsk::launch_batch(args, variant, grid, c->stream));
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    tm->launches += 1;
+    tm->items += (int64_t)(chunks[ci].second - chunks[ci].first) * p.n_k;
+  }
+  for (size_t ci = 0; host_out && ci < chunks.size(); ++ci) {
+    if (int rc = copy_rows(c, tiles, chunks[ci].first, chunks[ci].second, tr, p, self, elem,
+                           host_out, c->events[2 * ci + 1])) {
+      return rc;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipStreamSynchronize(c->copy_stream));
+  for (size_t ci = 0; ci < chunks.size(); ++ci) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->events[2 * ci], c->events[2 * ci + 1]));
+    tm->ms += ms;
+  }
+  return PPSK_OK;
+}
+
+// Dense pass over rows [row_begin, row_end): the persistent kernel for the
+// shapes it is built for (PPSK_PERSISTENT=0 selects the one-launch-per-(tile,
+// k) kernel), else the tile kernel.
+int run_dense(DeviceCtx* c, ppsk::DistParams p, int variant, bool upper, int64_t row_begin,
+              int64_t row_end, bool self, size_t elem, void* host_out, Timing* tm) {
+  const char* e = std::getenv("PPSK_PERSISTENT");
+  const bool persistent = (variant == 2 || variant == 4) && !(e && std::atoi(e) == 0);
+  if (persistent) {
+    return run_dense_persistent(c, p, variant, upper, row_begin, row_end, self, elem, host_out,
+                                tm);
+  }
+  return run_dense_tiles(c, p, variant, upper, row_begin, row_end, self, elem, host_out, tm);
 }
 
 void fill_common(ppsk::DistParams& p, const ppsk_set* rows, const ppsk_set* cols,

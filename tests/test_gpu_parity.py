@@ -187,11 +187,16 @@ def test_knn_asymmetric_table_band_path():
     assert max_abs_diff(dv, rd) <= CORE_ACC_TOL
 
 
+@pytest.mark.parametrize("persistent", ["1", "0"])
 @pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
-def test_every_tile_shape(monkeypatch, variant):
+def test_every_tile_shape(monkeypatch, variant, persistent):
     # every compiled tile configuration, self and query mode, against the
-    # oracle (sample counts not a multiple of any tile edge)
+    # oracle (sample counts not a multiple of any tile edge); shapes 2 and 4
+    # run both as the persistent kernel and as one launch per (tile, k)
+    if persistent == "1" and variant not in (2, 4):
+        pytest.skip("the persistent kernel is built for shapes 2 and 4 only")
     monkeypatch.setenv("PPSK_TILE_VARIANT", str(variant))
+    monkeypatch.setenv("PPSK_PERSISTENT", persistent)
     raw, le, _ = synth(300)
     with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
         c, _, _ = H.dense(s, out="counts")
