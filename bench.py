@@ -93,7 +93,7 @@ def measured_valu_peak():
 
 def traffic_from_profile():
     """HBM bytes per launch from the committed rocprofv3 counter summary."""
-    path = os.path.join(ROOT, "profiles", "dist_tiles_config2_traffic.json")
+    path = os.path.join(ROOT, "profiles", "dist_batch_allvsall_8192_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
