@@ -113,6 +113,8 @@ hipEvent_t pool_event(DeviceCtx* c, size_t i);
 // Distance parameters shared by every entry point (rows = r1 of the Jaccard).
 void fill_common(ppsk::DistParams& p, const ppsk_set* rows, const ppsk_set* cols,
                  const RandomIn& rnd, double tol);
+// 1 unless PPSK_ROW_PREFETCH=0 (row-panel L2 prefetch of the tile kernels).
+int row_prefetch();
 // Device copy of a Monte Carlo table (null for the other modes); caller frees.
 int upload_table(DeviceCtx* c, const ppsk_set* s, const RandomIn& rnd, float** dtab);
 

@@ -139,6 +139,11 @@ struct TileCfg {
   static constexpr int ROUNDS = (STAGE_U4 + WAVES * 64 - 1) / (WAVES * 64);  // 4
   static_assert(STAGE_U4 % 64 == 0, "stage must be whole wave rounds");
   static_assert(BLK % TROWS == 0, "row tiles must not straddle stripe blocks");
+  // row panel of one stage (WS words x 28 planes x TROWS samples) in uint4
+  static constexpr int ROW_U4 = WS * HL * TROWS / 4;
+  static constexpr int ROW_ROUNDS = (ROW_U4 + WAVES * 64 - 1) / (WAVES * 64);
+  // LDS target of the row prefetch: whole waves of uint4 (never read)
+  static constexpr int ROW_LDS_U4 = (ROW_U4 + 63) / 64 * 64;
 };
 
 // s_waitcnt vmcnt(n) with expcnt / lgkmcnt left unconstrained (gfx9 encoding)
@@ -220,6 +225,42 @@ __device__ __forceinline__ void issue_round(const uint32_t* sbase, const ColOffs
       dma16<I * WAVES * 64 * 16>(sbase, off.v[I], wbase);
     }
     issue_round<RQ, RR, I + 1>(sbase, off, wbase, wave);
+  }
+}
+
+// Row-panel prefetch: the stage's row planes (read later with scalar loads)
+// are copied into a scratch LDS area one stage ahead, only so that the scalar
+// loads find them in L2.  Lane offsets as for the columns.
+template <int RQ, int RR>
+struct RowOffsets {
+  uint32_t v[TileCfg<RQ, RR>::ROW_ROUNDS];
+};
+
+template <int RQ, int RR>
+__device__ __forceinline__ RowOffsets<RQ, RR> row_offsets(int wave, int lane) {
+  using Cfg = TileCfg<RQ, RR>;
+  constexpr int PER_PLANE = Cfg::TROWS / 4;  // uint4 per plane of the panel
+  RowOffsets<RQ, RR> o;
+#pragma unroll
+  for (int i = 0; i < Cfg::ROW_ROUNDS; ++i) {
+    int x = i * (WAVES * 64) + wave * 64 + lane;
+    if (x >= Cfg::ROW_U4) x = 0;  // lanes past the panel repeat its first uint4
+    const int plane = x / PER_PLANE;  // ws * HL + hl
+    const int part = x % PER_PLANE;
+    o.v[i] = (uint32_t)(plane * BLK + part * 4) * 4u;
+  }
+  return o;
+}
+
+template <int RQ, int RR, int I>
+__device__ __forceinline__ void prefetch_round(const uint32_t* rbase, const RowOffsets<RQ, RR>& off,
+                                               uint32_t wbase, int wave) {
+  using Cfg = TileCfg<RQ, RR>;
+  if constexpr (I < Cfg::ROW_ROUNDS) {
+    if (I * (WAVES * 64) + wave * 64 < Cfg::ROW_U4) {  // wave-uniform
+      dma16<I * WAVES * 64 * 16>(rbase, off.v[I], wbase);
+    }
+    prefetch_round<RQ, RR, I + 1>(rbase, off, wbase, wave);
   }
 }
 
@@ -558,7 +599,7 @@ __device__ __noinline__ void tile_epilogue(CParams* pp, const uint16_t* tile_cou
 // ---------------------------------------------------------------------------
 template <int RQ, int RR, int PB>
 __device__ __forceinline__ void tile_match(CParams* pp, Tile tile, int kk, uint4* sB0,
-                                           uint4* sB1, int wave, int lane,
+                                           uint4* sB1, uint4* sR, int wave, int lane,
                                            uint32_t (&cnt)[RQ][RR]) {
   using Cfg = TileCfg<RQ, RR>;
   constexpr int TC = Cfg::TC;
@@ -589,7 +630,14 @@ __device__ __forceinline__ void tile_match(CParams* pp, Tile tile, int kk, uint4
   const int n_stages = (s64 + WS - 1) / WS;
   const ColOffsets<RQ, RR> off = col_offsets<RQ, RR>(col0, nb_cols, s64, wave, lane);
   const uint32_t* cbase = col_kk + (int64_t)(col0 / BLK) * s64 * (HL * BLK);
+  const bool rpf = PB != 0 && pp->row_prefetch != 0;
+  const uint32_t* rbase = pp->row_stripes +
+                          ((int64_t)kk * pp->nb_rows + row0 / BLK) * (int64_t)s64 * (HL * BLK) +
+                          row0 % BLK;
+  const RowOffsets<RQ, RR> roff = row_offsets<RQ, RR>(wave, lane);
+  const uint32_t rdst = lds_addr(sR) + (uint32_t)(wave * 64 * 16);
   issue_stage<RQ, RR>(cbase, off, sB0, wave);
+  if (rpf) prefetch_round<RQ, RR, 0>(rbase, roff, rdst, wave);
   // One barrier per stage: after it, stage s is in LDS for every wave and
   // every wave has finished matching stage s - 1, whose buffer the copy of
   // stage s + 1 (issued right after the barrier) then overwrites while stage
@@ -608,6 +656,7 @@ __device__ __forceinline__ void tile_match(CParams* pp, Tile tile, int kk, uint4
         stage_barrier();
         if (s + 1 < n_stages) {
           issue_stage<RQ, RR>(cbase + (int64_t)(w0 + WS) * (HL * BLK), off, nxt, wave);
+          if (rpf) prefetch_round<RQ, RR, 0>(rbase + (int64_t)(w0 + WS) * (HL * BLK), roff, rdst, wave);
         }
         const int nw = (s64 - w0) < WS ? (s64 - w0) : WS;
         const uint32_t* sB = reinterpret_cast<const uint32_t*>(cur);
@@ -644,6 +693,7 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
   using Cfg = TileCfg<RQ, RR>;
   __shared__ uint4 sB0[Cfg::STAGE_U4];
   __shared__ uint4 sB1[Cfg::STAGE_U4];
+  __shared__ uint4 sR[PB ? Cfg::ROW_LDS_U4 : 64];  // row prefetch: pipelined shapes only
   __shared__ int s_last;
   // the parameters are the kernel's only argument: scalar loads straight
   // from the kernarg segment (no copy)
@@ -660,7 +710,7 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
   const int lane = threadIdx.x & 63;
 
   uint32_t cnt[RQ][RR];
-  tile_match<RQ, RR, PB>(pp, tile, kk, sB0, sB1, wave, lane, cnt);
+  tile_match<RQ, RR, PB>(pp, tile, kk, sB0, sB1, sR, wave, lane, cnt);
 
   // per-tile count scratch: [tile][kk][row][col]
   uint16_t* tile_counts = p.counts + t * p.n_k * (int64_t)(Cfg::TROWS * Cfg::TC);
@@ -701,6 +751,7 @@ __device__ __noinline__ void batch_tile_k(CParams* pp_in, int bi_in, int ct_in, 
   using Cfg = TileCfg<RQ, RR>;
   __shared__ uint4 sB0[Cfg::STAGE_U4];
   __shared__ uint4 sB1[Cfg::STAGE_U4];
+  __shared__ uint4 sR[Cfg::ROW_LDS_U4];
   const uint64_t pv = (uint64_t)(uintptr_t)pp_in;
   const uint32_t plo = __builtin_amdgcn_readfirstlane((uint32_t)pv);
   const uint32_t phi = __builtin_amdgcn_readfirstlane((uint32_t)(pv >> 32));
@@ -716,7 +767,7 @@ __device__ __noinline__ void batch_tile_k(CParams* pp_in, int bi_in, int ct_in, 
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   uint32_t cnt[RQ][RR];
-  tile_match<RQ, RR, PB>(pp, tile, kk, sB0, sB1, wave, lane, cnt);
+  tile_match<RQ, RR, PB>(pp, tile, kk, sB0, sB1, sR, wave, lane, cnt);
   store_counts<RQ, RR>(scratch, kk, wave, lane, cnt);
 }
 

@@ -392,12 +392,23 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
 }
 
 // Dense pass over rows [row_begin, row_end): the persistent kernel for the
-// shapes it is built for (PPSK_PERSISTENT=0 selects the one-launch-per-(tile,
-// k) kernel), else the tile kernel.
+// shapes it is built for when the pass has at least two tiles per resident
+// workgroup (a smaller pass fills the GPU better split per (tile, k-mer
+// length)); PPSK_PERSISTENT=0 / 1 forces the choice.
 int run_dense(DeviceCtx* c, ppsk::DistParams p, int variant, bool upper, int64_t row_begin,
               int64_t row_end, bool self, size_t elem, void* host_out, Timing* tm) {
   const char* e = std::getenv("PPSK_PERSISTENT");
-  const bool persistent = (variant == 2 || variant == 4) && !(e && std::atoi(e) == 0);
+  bool persistent = variant == 2 || variant == 4;
+  if (e) {
+    persistent = persistent && std::atoi(e) != 0;
+  } else if (persistent) {
+    const int tr = ppsk::variant_tile_rows(variant), tc = ppsk::variant_tile_cols(variant);
+    const int64_t rt = (row_end - row_begin + tr - 1) / tr, ct = (p.n_cols + tc - 1) / tc;
+    const int64_t tiles = upper ? rt * ct / 2 : rt * ct;
+    int grid = 0;
+    if (ppsk::batch_grid(variant, &grid) != hipSuccess) grid = 0;
+    persistent = tiles >= 2 * (int64_t)grid;
+  }
   if (persistent) {
     return run_dense_persistent(c, p, variant, upper, row_begin, row_end, self, elem, host_out,
                                 tm);
@@ -424,6 +435,14 @@ void fill_common(ppsk::DistParams& p, const ppsk_set* rows, const ppsk_set* cols
   p.jv_cols = cols->jv;
   p.cl_rows = rows->clusters;
   p.cl_cols = cols->clusters;
+  p.row_prefetch = row_prefetch();
+}
+
+// Row-panel L2 prefetch in the pipelined tile loop: on unless
+// PPSK_ROW_PREFETCH=0.
+int row_prefetch() {
+  const char* e = std::getenv("PPSK_ROW_PREFETCH");
+  return e ? (std::atoi(e) != 0) : 1;
 }
 
 int upload_table(DeviceCtx* c, const ppsk_set* s, const RandomIn& rnd, float** dtab) {

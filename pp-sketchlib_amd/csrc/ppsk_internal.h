@@ -69,6 +69,7 @@ struct DistParams {
   unsigned long long* fail_count;
   unsigned long long* fail_first;  // smallest failing dense row
   uint32_t* fail_rows;             // optional [n_rows] failures per tile row sample
+  int32_t row_prefetch;            // pull each stage's row panel into L2 a stage ahead
 };
 
 // Persistent batch kernel (dist_batch): work queues and items.

@@ -269,6 +269,7 @@ int run_step(ppsk_server* s, Worker& w, const std::vector<ppsk_request*>& batch,
   p.jv_rows = s->random_mode == PPSK_RANDOM_EQUAL_BASE ? jv : nullptr;
   p.jv_cols = ref->jv;
   p.mc_ref_is_col = 1;
+  p.row_prefetch = row_prefetch();
   p.out_kind = PPSK_OUT_CORE_ACC;
   p.out = band;
   p.counts = scratch;

@@ -211,3 +211,13 @@ def test_every_tile_shape(monkeypatch, variant, persistent):
     assert np.array_equal(c, ref_c)
     assert np.abs(d - ref).max() <= CORE_ACC_TOL
     assert np.abs(qd - qref).max() <= CORE_ACC_TOL
+
+
+def test_row_prefetch_off_is_identical(monkeypatch):
+    # the row-panel L2 prefetch only moves data earlier: outputs are unchanged
+    raw, le, _ = synth(300)
+    with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
+        on, _, _ = H.dense(s, random_mode=H.RANDOM_EQUAL_BASE)
+        monkeypatch.setenv("PPSK_ROW_PREFETCH", "0")
+        off, _, _ = H.dense(s, random_mode=H.RANDOM_EQUAL_BASE)
+    assert np.array_equal(on.view(np.uint32), off.view(np.uint32))
