@@ -406,6 +406,27 @@ __device__ __forceinline__ void batch_wait(Batch<RQ, RR, PB>& x) {
   }
 }
 
+// Keeps a batch's registers alive (unchanged) at this point of the program.
+// Placed right after the next batch's loads are issued, it stops the
+// compiler from matching the current batch first and then reusing its
+// registers for the next batch's loads, which would serialise every load
+// round trip behind the matching.
+template <int RQ, int RR, int PB>
+__device__ __forceinline__ void batch_hold(Batch<RQ, RR, PB>& x) {
+  if constexpr (PB == 2 && RR == 4) {
+    asm volatile(""
+                 : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0][0]), "+v"(x.b[0][1]),
+                   "+v"(x.b[1][0]), "+v"(x.b[1][1]));
+  } else if constexpr (PB == 2 && RR == 2) {
+    asm volatile("" : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0][0]), "+v"(x.b[1][0]));
+  } else if constexpr (PB == 1 && RR == 4) {
+    asm volatile("" : "+s"(x.a[0]), "+v"(x.b[0][0]), "+v"(x.b[0][1]));
+  } else {
+    static_assert(PB == 1 && RR == 2, "unsupported batch shape");
+    asm volatile("" : "+s"(x.a[0]), "+v"(x.b[0][0]));
+  }
+}
+
 // Planes P .. P+PB-1 of one word against the lane's RR columns.
 template <int RQ, int RR, int PB, int P>
 __device__ __forceinline__ void match_batch(const Batch<RQ, RR, PB>& x, uint32_t (&acc)[RQ][RR]) {
@@ -440,6 +461,7 @@ __device__ __forceinline__ void match_batches(const CU32* aw, uint32_t bw, bool 
   } else {
     if (next_word) batch_issue<RQ, RR, PB, HL>(aw, bw, nxt);  // word w + 1
   }
+  batch_hold<RQ, RR, PB>(cur);
   match_batch<RQ, RR, PB, B * PB>(cur, acc);
   if constexpr (((B + 1) * PB) % BBITS == 0) {
     // end of a 32-bit half: fold its matches into the counts
