@@ -121,6 +121,8 @@ struct ppsk_server {
   float cutoff = 0;
   int32_t max_rows = 0;
   int variant = 0;  // tile shape; 0: by step size (step_variant)
+  int grid2 = 0, grid4 = 0;  // resident workgroups of the persistent kernel per shape
+  bool force_persistent = false;  // PPSK_PERSISTENT=1: every step of shape 2 / 4
   size_t per_sample = 0;
   size_t scratch_budget = 0;
   Arena arena;
@@ -150,9 +152,16 @@ int step_variant(const ppsk_server* s, int64_t rows) {
   return 2;
 }
 
+// Device memory of one step, carved from the arena.  A step with at least
+// two tiles per resident workgroup runs the persistent tile kernel (whole
+// tiles from work queues, per-workgroup count blocks); a smaller one is split
+// per (tile, k-mer length) over more workgroups (per-tile count scratch,
+// launched in chunks of chunk_tiles).
 struct StepLayout {
-  size_t staging, stripes, jv, band, tiles, scratch, tile_done, fail, fail_rows, total;
+  size_t staging, stripes, jv, band, tiles, scratch, tile_done, fail, fail_rows, params, total;
   int64_t nbq, n_tiles, chunk_tiles;
+  bool persistent;
+  int grid;
 };
 
 StepLayout layout(const ppsk_server* s, int64_t rows) {
@@ -164,6 +173,8 @@ StepLayout layout(const ppsk_server* s, int64_t rows) {
   L.nbq = (rows + ppsk::BLK - 1) / ppsk::BLK;
   L.n_tiles = ((rows + tr - 1) / tr) * ((r->n + tc - 1) / tc);
   const size_t per_tile = (size_t)r->n_k * tr * tc * sizeof(uint16_t);
+  L.grid = v == 2 ? s->grid2 : (v == 4 ? s->grid4 : 0);
+  L.persistent = L.grid > 0 && (s->force_persistent || L.n_tiles >= 2 * (int64_t)L.grid);
   L.chunk_tiles = std::max<int64_t>(1, std::min<int64_t>(L.n_tiles, s->scratch_budget / per_tile));
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -175,9 +186,17 @@ StepLayout layout(const ppsk_server* s, int64_t rows) {
   L.stripes = take((size_t)r->n_k * L.nbq * r->s64 * ppsk::HL * ppsk::BLK * sizeof(uint32_t));
   L.jv = take((size_t)rows * r->n_k * sizeof(double));
   L.band = take((size_t)rows * r->n * 2 * sizeof(float));
-  L.tiles = take((size_t)L.n_tiles * sizeof(ppsk::Tile));
-  L.scratch = take((size_t)L.chunk_tiles * per_tile);
-  L.tile_done = take((size_t)L.chunk_tiles * sizeof(uint32_t));
+  if (L.persistent) {
+    L.tiles = take((size_t)L.n_tiles * sizeof(ppsk::BatchItem));
+    L.scratch = take((size_t)L.grid * per_tile);
+    L.tile_done = take(ppsk::BATCH_QUEUES * sizeof(uint32_t));  // queue counters
+    L.params = take(sizeof(ppsk::DistParams));
+  } else {
+    L.tiles = take((size_t)L.n_tiles * sizeof(ppsk::Tile));
+    L.scratch = take((size_t)L.chunk_tiles * per_tile);
+    L.tile_done = take((size_t)L.chunk_tiles * sizeof(uint32_t));
+    L.params = 0;
+  }
   L.fail = take(2 * sizeof(unsigned long long));
   L.fail_rows = take((size_t)rows * sizeof(uint32_t));
   L.total = off;
@@ -282,17 +301,46 @@ int run_step(ppsk_server* s, Worker& w, const std::vector<ppsk_request*>& batch,
   const int tc = ppsk::variant_tile_cols(variant);
   std::vector<size_t> gs;
   const auto tiles = make_tiles(rows, ref->n, tr, tc, false, 0, rows, tile_group(), &gs);
-  STEP_TRY(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(ppsk::Tile),
-                          hipMemcpyHostToDevice, w.st));
-  STEP_TRY(hipEventRecord(w.e0, w.st));
-  for (size_t a = 0; a < tiles.size(); a += (size_t)L.chunk_tiles) {
-    const size_t b = std::min(tiles.size(), a + (size_t)L.chunk_tiles);
-    p.tiles = d_tiles + a;
-    p.n_tiles = (int64_t)(b - a);
-    p.item_base = 0;
-    STEP_TRY(ppsk::launch_dists(p, variant, (int64_t)(b - a) * n_k, w.st));
+  if (L.persistent) {
+    // whole tiles from 8 queues, each a contiguous eighth of the tile list
+    std::vector<ppsk::BatchItem> items;
+    items.reserve(tiles.size());
+    ppsk::BatchArgs args;
+    std::memset(&args, 0, sizeof(args));
+    const size_t n = tiles.size();
+    for (int q = 0; q < ppsk::BATCH_QUEUES; ++q) {
+      args.queue_begin[q] = (int64_t)items.size();
+      for (size_t x = n * q / ppsk::BATCH_QUEUES; x < n * (q + 1) / ppsk::BATCH_QUEUES; ++x) {
+        items.push_back({0, tiles[x].bi, tiles[x].ct, 0});
+      }
+    }
+    args.queue_begin[ppsk::BATCH_QUEUES] = (int64_t)items.size();
+    ppsk::DistParams* d_params = (ppsk::DistParams*)(mem + L.params);
+    args.jobs = d_params;
+    args.items = (const ppsk::BatchItem*)(mem + L.tiles);
+    args.queue_next = tile_done;
+    args.scratch = scratch;
+    args.max_k = n_k;
+    STEP_TRY(hipMemcpyAsync(mem + L.tiles, items.data(), items.size() * sizeof(ppsk::BatchItem),
+                            hipMemcpyHostToDevice, w.st));
+    STEP_TRY(hipMemcpyAsync(d_params, &p, sizeof(p), hipMemcpyHostToDevice, w.st));
+    STEP_TRY(ppsk::launch_batch_reset(tile_done, nullptr, 0, w.st));
+    STEP_TRY(hipEventRecord(w.e0, w.st));
+    STEP_TRY(ppsk::launch_batch(args, variant, L.grid, w.st));
+    STEP_TRY(hipEventRecord(w.e1, w.st));
+  } else {
+    STEP_TRY(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(ppsk::Tile),
+                            hipMemcpyHostToDevice, w.st));
+    STEP_TRY(hipEventRecord(w.e0, w.st));
+    for (size_t a = 0; a < tiles.size(); a += (size_t)L.chunk_tiles) {
+      const size_t b = std::min(tiles.size(), a + (size_t)L.chunk_tiles);
+      p.tiles = d_tiles + a;
+      p.n_tiles = (int64_t)(b - a);
+      p.item_base = 0;
+      STEP_TRY(ppsk::launch_dists(p, variant, (int64_t)(b - a) * n_k, w.st));
+    }
+    STEP_TRY(hipEventRecord(w.e1, w.st));
   }
-  STEP_TRY(hipEventRecord(w.e1, w.st));
 
   // kNN and threshold counts straight into page-locked host memory
   const size_t nk = (size_t)rows * s->kNN;
@@ -470,6 +518,12 @@ int ppsk_server_create(ppsk_set* ref, int32_t random_mode, double tolerance, int
   s->max_rows = max_rows;
   s->variant = std::getenv("PPSK_TILE_VARIANT") ? default_variant() : 0;
   s->per_sample = (size_t)ref->n_k * ref->s64 * ppsk::BBITS * sizeof(uint64_t);
+  if (ppsk::batch_grid(2, &s->grid2) != hipSuccess) s->grid2 = 0;
+  if (ppsk::batch_grid(4, &s->grid4) != hipSuccess) s->grid4 = 0;
+  if (const char* e = std::getenv("PPSK_PERSISTENT")) {
+    if (std::atoi(e) == 0) s->grid2 = s->grid4 = 0;
+    s->force_persistent = std::atoi(e) == 1;
+  }
   s->scratch_budget = (size_t)256 << 20;
   if (arena_bytes == 0) {
     size_t fr = 0, tot = 0;

@@ -55,7 +55,11 @@ def _same(a, b):
     assert a["n_fail"] == b["n_fail"]
 
 
-def test_request_same_alone_and_in_mix(data):
+@pytest.mark.parametrize("persistent", ["0", "1"])
+def test_request_same_alone_and_in_mix(data, monkeypatch, persistent):
+    # steps run either split per (tile, k-mer length) or on the persistent
+    # kernel (large steps choose it by themselves; forced here)
+    monkeypatch.setenv("PPSK_PERSISTENT", persistent)
     ref, q_raw, q_le = data["ref"], data["q_raw"], data["q_le"]
     a0, a1 = 100, 137  # the request under test: 37 queries
     with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
