@@ -33,41 +33,63 @@ namespace ppsk {
 // ---------------------------------------------------------------------------
 // Stripe transform: raw reference layout -> stripes.
 // raw: [count][n_k][s64 * 14] uint64 (word w, layer l at w*14 + l,
-// sketch/sketch.cpp:82-92).  One thread per (sample, kk, w).
+// sketch/sketch.cpp:82-92), samples first .. first+count-1 of the set.
+// One workgroup per (kk, stripe block, w): the block's 128 sample records
+// of 14 uint64 (112 B each, contiguous in raw) are read with 16-byte loads
+// into LDS, then the (kk, block, w) slab of 28 planes x 128 samples is
+// written as one contiguous 14 KiB run.  Samples of the block outside
+// [first, first + count) are left untouched (an upload in several chunks).
 // ---------------------------------------------------------------------------
-__global__ void stripe_kernel(const uint64_t* __restrict__ raw, int64_t count,
-                              int64_t first, int32_t n_k, int32_t s64, int32_t nb,
-                              uint32_t* __restrict__ stripes) {
-  const int64_t total = count * n_k * s64;
-  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total;
-       x += (int64_t)gridDim.x * blockDim.x) {
-    // sample-fastest so that stores of one plane are coalesced
-    const int64_t s = x % count;
-    const int64_t rest = x / count;
-    const int32_t w = (int32_t)(rest % s64);
-    const int32_t kk = (int32_t)(rest / s64);
-    const uint64_t* src = raw + ((s * n_k + kk) * (int64_t)s64 + w) * BBITS;
-    const int64_t n = first + s;
-    const int64_t blk = n / BLK, lane = n % BLK;
-    uint32_t* dst = stripes + (((int64_t)kk * nb + blk) * s64 + w) * (HL * BLK) + lane;
-#pragma unroll
-    for (int l = 0; l < BBITS; ++l) {
-      const uint64_t v = src[l];
-      dst[l * BLK] = (uint32_t)v;
-      dst[(BBITS + l) * BLK] = (uint32_t)(v >> 32);
+constexpr int STRIPE_TB = 256;
+constexpr int STRIPE_REC = BBITS + 1;  // uint64 per LDS record (one pad word)
+
+__global__ __launch_bounds__(STRIPE_TB) void stripe_kernel(const uint64_t* __restrict__ raw,
+                                                           int64_t count, int64_t first,
+                                                           int32_t n_k, int32_t s64, int32_t nb,
+                                                           uint32_t* __restrict__ stripes) {
+  __shared__ uint64_t rec[BLK * STRIPE_REC];
+  const int64_t blk = (int64_t)first / BLK + blockIdx.y;
+  const int kk = (int)(blockIdx.z);
+  for (int w = (int)blockIdx.x; w < s64; w += (int)gridDim.x) {
+    // samples of this block in this upload: n in [n0, n1)
+    const int64_t n0 = blk * BLK > first ? blk * BLK : first;
+    const int64_t n1 = (blk + 1) * BLK < first + count ? (blk + 1) * BLK : first + count;
+    // load: 7 uint4 per sample record
+    for (int x = threadIdx.x; x < BLK * 7; x += STRIPE_TB) {
+      const int lane = x / 7, part = x % 7;
+      const int64_t n = blk * BLK + lane;
+      if (n >= n0 && n < n1) {
+        const uint4* src = reinterpret_cast<const uint4*>(
+            raw + (((n - first) * n_k + kk) * (int64_t)s64 + w) * BBITS);
+        const uint4 v = src[part];
+        uint64_t* d = rec + lane * STRIPE_REC + part * 2;
+        d[0] = (uint64_t)v.x | ((uint64_t)v.y << 32);
+        d[1] = (uint64_t)v.z | ((uint64_t)v.w << 32);
+      }
     }
+    __syncthreads();
+    uint32_t* dst = stripes + (((int64_t)kk * nb + blk) * s64 + w) * (HL * BLK);
+    for (int o = threadIdx.x; o < HL * BLK; o += STRIPE_TB) {
+      const int hl = o / BLK, lane = o % BLK;
+      const int64_t n = blk * BLK + lane;
+      if (n >= n0 && n < n1) {
+        const int h = hl / BBITS, l = hl % BBITS;
+        const uint64_t v = rec[lane * STRIPE_REC + l];
+        dst[o] = h ? (uint32_t)(v >> 32) : (uint32_t)v;
+      }
+    }
+    __syncthreads();
   }
 }
 
 hipError_t launch_stripe(const uint64_t* raw, int64_t count, int64_t first,
                          int32_t n_k, int32_t s64, int32_t nb, uint32_t* stripes,
                          hipStream_t stream) {
-  const int64_t total = count * n_k * s64;
-  if (total == 0) return hipSuccess;
-  int64_t blocks = (total + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  stripe_kernel<<<(unsigned)blocks, 256, 0, stream>>>(raw, count, first, n_k, s64, nb,
-                                                        stripes);
+  if (count <= 0) return hipSuccess;
+  const int64_t b0 = first / BLK, b1 = (first + count - 1) / BLK;
+  const unsigned words = (unsigned)(s64 < 64 ? s64 : 64);  // grid x strides over words
+  dim3 grid(words, (unsigned)(b1 - b0 + 1), (unsigned)n_k);
+  stripe_kernel<<<grid, STRIPE_TB, 0, stream>>>(raw, count, first, n_k, s64, nb, stripes);
   return hipGetLastError();
 }
 

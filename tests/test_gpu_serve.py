@@ -139,3 +139,26 @@ def test_server_rejects_bad_requests(data):
             srv.submit(q_raw[:2])
         with pytest.raises(H.PpskError, match="ticket"):
             srv.wait(12345)
+
+
+def test_admission_by_memory(data):
+    # an arena that holds about one 64-query step: with 3 slots the workers
+    # must wait for device memory, never exceed the arena, and still return
+    # the same results as requests served alone
+    ref, q_raw, q_le = data["ref"], data["q_raw"], data["q_le"]
+    arena = 48 << 20
+    spans = [(x * 64, x * 64 + 64) for x in range(8)]
+    with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
+                  max_rows=64, n_slots=3) as solo:
+        alone = [solo.wait(solo.submit(q_raw[s:e], q_le[s:e])) for s, e in spans]
+    with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
+                  max_rows=64, n_slots=3, arena_bytes=arena) as srv:
+        tickets = [srv.submit(q_raw[s:e], q_le[s:e]) for s, e in spans]
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert 0 < st["peak_arena_bytes"] <= arena
+    assert st["peak_inflight"] == 1  # two 64-query steps never fit together
+    for a, b in zip(alone, res):
+        _same(a, b)
+    with pytest.raises(H.PpskError, match="arena"):
+        H.Server(ref, kNN=KNN, cutoff=CUTOFF, max_rows=64, n_slots=1, arena_bytes=1 << 20)
