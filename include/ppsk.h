@@ -298,6 +298,28 @@ int ppsk_server_free(ppsk_server *server);
 int ppsk_last_timing(double *kernel_ms, int64_t *n_launches, int64_t *n_items);
 
 /*
+ * Sketch sequences on the device: the bindash sketch over canonical
+ * (use_rc) or forward ntHash values of every k-mer free of non-ACGTU
+ * characters, densification and bit slicing, as sketch()
+ * (src/sketch/sketch.cpp:118-176) with dense (not codon-phased) seeds and no
+ * read filtering.  seq holds every sample's contigs back to back; contig c
+ * is seq[contig_off[c] .. contig_off[c+1]); sample s owns contigs
+ * sample_contig[s] .. sample_contig[s+1]-1 (contigs shorter than k yield no
+ * k-mers; the reference's reader already drops contigs shorter than the
+ * largest k, src/api.cpp:87).  raw_out: [n_samples][n_k][s64 * 14] words in
+ * the reference layout; inv_minhash (may be NULL): per sample the sum over k
+ * of bin 0 / (2^61 - 1) before densification (the reads size estimate is
+ * n_k / that sum, src/reference.cpp:49-55); densified (may be NULL): per
+ * sample 1 if any k needed densification.  Replaces create_sketches /
+ * sketch_gpu (src/api.cpp:39-138, src/sketch/sketch.cpp:178-205).
+ */
+int ppsk_sketch(int device_id, int64_t n_samples, const char *seq,
+                int64_t seq_bytes, const int64_t *contig_off,
+                int64_t n_contigs, const int64_t *sample_contig, int32_t n_k,
+                const int32_t *klist, int32_t sketchsize64, int32_t use_rc,
+                uint64_t *raw_out, double *inv_minhash, int32_t *densified);
+
+/*
  * Seeded synthetic sketches (no reference counterpart: a generator for tests
  * and benchmarks, writing the reference word layout directly).  Bins evolve
  * down a random binary tree: a child keeps its parent's 14-bit bin value with

@@ -448,6 +448,116 @@ void orc_knn_rows(const float *rows, const long *row_ids, long long m, long long
 }
 
 /* ---------------------------------------------------------------------------
+ * Sketching -- sketch/sketch.cpp:22-176 (bindash one-permutation sketch),
+ * sketch/nthash.hpp:17-127 and stHashIterator.hpp:56-116 (canonical ntHash).
+ *
+ * ntHash in its split-rotation form (Mohamadi et al. 2016): rot(x) rotates
+ * the high 31 bits and the low 33 bits of x left by one position each.  For
+ * a k-mer s_0 .. s_{k-1},
+ *   fh = XOR_i rot^(k-1-i)(seed(s_i)),   rh = XOR_i rot^i(seed(comp(s_i))),
+ * and the hash is min(fh, rh) (canonical) or fh.  k-mers containing any
+ * character other than A, C, G, T, U (either case) are skipped.  The seeds
+ * are the published ntHash constants (nthash_tables.hpp:22-25).
+ * Each hash is reduced mod 2^61 - 1; bin = hash / ceil((2^61 - 1) / nbins)
+ * keeps its minimum; empty bins are filled by universal-hash probing
+ * (densifybin, sketch.cpp:93-115); bit i of each bin minimum goes to bit
+ * (bin % 64) of layer word (bin / 64) * 14 + i (fillusigs, sketch.cpp:80-91).
+ * ------------------------------------------------------------------------- */
+static const uint64_t ORC_SIGN_MOD = (1ULL << 61) - 1ULL;
+
+static uint64_t orc_seed(unsigned char c) {
+  switch (c) {
+    case 'A': case 'a': return 0x3c8bfbb395c60474ULL;
+    case 'C': case 'c': return 0x3193c18562a02b4cULL;
+    case 'G': case 'g': return 0x20323ed082572324ULL;
+    case 'T': case 't': case 'U': case 'u': return 0x295549f54be24456ULL;
+    default: return 0;
+  }
+}
+
+static uint64_t orc_seed_comp(unsigned char c) {
+  switch (c) {
+    case 'A': case 'a': return orc_seed('T');
+    case 'C': case 'c': return orc_seed('G');
+    case 'G': case 'g': return orc_seed('C');
+    case 'T': case 't': case 'U': case 'u': return orc_seed('A');
+    default: return 0;
+  }
+}
+
+/* rot^j: rotate the high 31 bits and the low 33 bits left by j each */
+static uint64_t orc_rot(uint64_t x, unsigned j) {
+  uint64_t hi = x >> 33, lo = x & 0x1FFFFFFFFULL;
+  const unsigned a = j % 31, b = j % 33;
+  hi = a ? (((hi << a) | (hi >> (31 - a))) & 0x7FFFFFFFULL) : hi;
+  lo = b ? (((lo << b) | (lo >> (33 - b))) & 0x1FFFFFFFFULL) : lo;
+  return (hi << 33) | lo;
+}
+
+static uint64_t orc_univhash(uint64_t s, uint64_t t) {
+  uint64_t x = 1009ULL * s + (1000ULL * 1000ULL + 3ULL) * t;
+  return (48271ULL * x + 11ULL) % ((1ULL << 31) - 1ULL);
+}
+
+/* One sample, one k: seq holds the contigs back to back, contig c at
+ * [off[c], off[c+1]).  usigs: [s64 * 14].  Returns the densify code
+ * (0 none, 1 densified, -1 every bin empty); *inv_minhash = bin 0 / (2^61-1)
+ * before densification (sketch.cpp:72-75). */
+int orc_sketch(const char *seq, const long long *off, int n_contigs, int k, int s64,
+               int use_rc, uint64_t *usigs, double *inv_minhash) {
+  const uint64_t nbins = (uint64_t)s64 * 64;
+  const uint64_t binsize = (ORC_SIGN_MOD + nbins - 1ULL) / nbins;
+  std::vector<uint64_t> signs(nbins, UINT64_MAX);
+  for (int c = 0; c < n_contigs; ++c) {
+    const long long a = off[c], len = off[c + 1] - off[c];
+    for (long long p = 0; p + k <= len; ++p) {
+      uint64_t fh = 0, rh = 0;
+      bool ok = true;
+      for (int i = 0; i < k; ++i) {
+        const unsigned char ch = (unsigned char)seq[a + p + i];
+        const uint64_t sd = orc_seed(ch);
+        if (!sd) { ok = false; break; }
+        fh ^= orc_rot(sd, (unsigned)(k - 1 - i));
+        rh ^= orc_rot(orc_seed_comp(ch), (unsigned)i);
+      }
+      if (!ok) continue;
+      const uint64_t h = ((use_rc && rh < fh) ? rh : fh) % ORC_SIGN_MOD;
+      const uint64_t bin = h / binsize;
+      if (h < signs[bin]) signs[bin] = h;
+    }
+  }
+  *inv_minhash = signs[0] / (double)ORC_SIGN_MOD;
+  uint64_t mn = UINT64_MAX, mx = 0;
+  for (uint64_t v : signs) {
+    mn = v < mn ? v : mn;
+    mx = v > mx ? v : mx;
+  }
+  int dens = 0;
+  if (mx == UINT64_MAX) {
+    if (mn == UINT64_MAX) {
+      dens = -1;
+    } else {
+      for (uint64_t i = 0; i < nbins; ++i) {
+        uint64_t j = i, attempts = 0;
+        while (signs[j] == UINT64_MAX) {
+          j = orc_univhash(i, attempts) % nbins;
+          attempts++;
+        }
+        signs[i] = signs[j];
+      }
+      dens = 1;
+    }
+  }
+  std::fill(usigs, usigs + (size_t)s64 * 14, 0ULL);
+  for (uint64_t b = 0; b < nbins; ++b) {
+    for (int i = 0; i < 14; ++i) {
+      usigs[(b / 64) * 14 + i] |= ((signs[b] >> i) & 1ULL) << (b % 64);
+    }
+  }
+  return dens;
+}
+
+/* ---------------------------------------------------------------------------
  * Index math -- dist/matrix_idx.hpp:13-50.
  * ------------------------------------------------------------------------- */
 long long orc_square_to_condensed(long i, long j, long n) { return sq2cond(i, j, n); }

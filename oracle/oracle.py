@@ -57,6 +57,9 @@ def _load():
                                         _c.c_int, _p, _p, _p]
     lib.orc_knn_from_square.restype = None
     lib.orc_knn_from_square.argtypes = [_p, _ll, _c.c_int, _p, _p, _p, _c.c_int]
+    lib.orc_sketch.restype = _c.c_int
+    lib.orc_sketch.argtypes = [_c.c_char_p, _p, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _p,
+                               _c.POINTER(_c.c_double)]
     lib.orc_knn_rows.restype = None
     lib.orc_knn_rows.argtypes = [_p, _p, _ll, _ll, _c.c_int, _p, _p, _p, _c.c_int]
     lib.orc_square_to_condensed.restype = _ll
@@ -249,6 +252,67 @@ def knn_from_square(square, kNN, num_threads=None):
     L.orc_knn_from_square(_ptr(sq), n, int(kNN), _ptr(iv), _ptr(jv), _ptr(dv),
                           num_threads or L.orc_max_threads())
     return iv, jv, dv
+
+
+def sketch(contigs, k, sketchsize64, use_rc=True):
+    """sketch/sketch.cpp:118-176 for one sample and one k.  contigs: list of
+    str/bytes.  Returns (usigs [s64*14] uint64, inv_minhash, densify code)."""
+    L = lib()
+    bs = [c.encode() if isinstance(c, str) else bytes(c) for c in contigs]
+    seq = b"".join(bs)
+    off = np.zeros(len(bs) + 1, np.int64)
+    off[1:] = np.cumsum([len(b) for b in bs])
+    us = np.zeros(sketchsize64 * 14, np.uint64)
+    mh = _c.c_double()
+    dens = L.orc_sketch(seq, _ptr(off), len(bs), int(k), int(sketchsize64), int(bool(use_rc)),
+                        _ptr(us), _c.byref(mh))
+    return us, mh.value, dens
+
+
+_REF_SKETCH = None
+
+
+def ref_sketch_lib():
+    """The reference's own sketch core (oracle/_ref), or None if not built."""
+    global _REF_SKETCH
+    if _REF_SKETCH is None:
+        path = os.path.join(_HERE, "_ref", "libref_sketch.so")
+        if not os.path.exists(path):
+            return None
+        L = _c.CDLL(path)
+        L.ref_sketch_seqs.argtypes = [_c.POINTER(_c.c_char_p), _c.c_int, _c.c_int, _c.c_int,
+                                      _c.c_int, _p, _c.POINTER(_c.c_double),
+                                      _c.POINTER(_c.c_int)]
+        L.ref_read_stats.argtypes = [_c.c_char_p, _c.c_int, _p, _c.POINTER(_c.c_uint64),
+                                     _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_int),
+                                     _c.POINTER(_c.c_uint64)]
+        _REF_SKETCH = L
+    return _REF_SKETCH
+
+
+def ref_sketch(contigs, k, sketchsize64, use_rc=True):
+    """The reference's compiled sketch() on the same input."""
+    L = ref_sketch_lib()
+    bs = [c.encode() if isinstance(c, str) else bytes(c) for c in contigs]
+    arr = (_c.c_char_p * len(bs))(*bs)
+    us = np.zeros(sketchsize64 * 14, np.uint64)
+    mh = _c.c_double()
+    dn = _c.c_int()
+    if L.ref_sketch_seqs(arr, len(bs), int(k), int(sketchsize64), int(bool(use_rc)), _ptr(us),
+                         _c.byref(mh), _c.byref(dn)):
+        raise RuntimeError("reference sketch failed")
+    return us, mh.value, dn.value
+
+
+def ref_read_stats(path, kmer_len):
+    L = ref_sketch_lib()
+    b = np.zeros(4, np.float64)
+    tot, nc, ns = _c.c_uint64(), _c.c_uint64(), _c.c_uint64()
+    rd = _c.c_int()
+    if L.ref_read_stats(path.encode(), int(kmer_len), _ptr(b), _c.byref(tot), _c.byref(nc),
+                        _c.byref(rd), _c.byref(ns)):
+        raise RuntimeError("reference reader failed")
+    return b, tot.value, nc.value, bool(rd.value), ns.value
 
 
 def knn_rows(rows, row_ids, kNN, num_threads=None):

@@ -1,0 +1,57 @@
+/*
+ * C entry points around the reference's own sketching core, built from its
+ * sources with -DWEB_SKETCH (sketch/sketch.cpp, sketch/seqio.cpp,
+ * sketch/countmin.cpp).  Test infrastructure: lets the oracle's restatement
+ * and the HIP sketcher be cross-checked against the reference's compiled
+ * bindash sketch and sequence reader.
+ */
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sketch/sketch.hpp"
+
+// Sketch of one sample given as n_seqs in-memory sequences (one per
+// contig), one k-mer length, dense (not codon-phased) seeds.  usigs:
+// [s64 * 14].  Returns 0, or -1 if the reference threw.
+extern "C" int ref_sketch_seqs(const char *const *seqs, int n_seqs, int k, int s64,
+                               int use_rc, uint64_t *usigs, double *inv_minhash,
+                               int *densified) {
+  try {
+    std::vector<std::string> in;
+    for (int i = 0; i < n_seqs; ++i) in.emplace_back(seqs[i]);
+    SeqBuf buf(in);
+    KmerSeeds seeds = generate_seeds({(size_t)k}, false);
+    auto res = sketch(buf, (uint64_t)s64, seeds[(size_t)k], 14, false, use_rc != 0, 0, false);
+    const std::vector<uint64_t> &u = std::get<0>(res);
+    std::memcpy(usigs, u.data(), u.size() * sizeof(uint64_t));
+    *inv_minhash = std::get<1>(res);
+    *densified = std::get<2>(res) ? 1 : 0;
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+// The reference's file reader (kseq over zlib): composition of the first
+// 1M bases, total bases, N count, read flag and the number of sequences kept
+// (those at least kmer_len long).
+extern "C" int ref_read_stats(const char *path, int kmer_len, double *bases4, uint64_t *total,
+                              uint64_t *n_count, int *reads, uint64_t *n_seqs) {
+  try {
+    SeqBuf buf(std::vector<std::string>{path}, (size_t)kmer_len);
+    BaseComp<double> b = buf.get_composition();
+    bases4[0] = b.a;
+    bases4[1] = b.c;
+    bases4[2] = b.g;
+    bases4[3] = b.t;
+    *total = (uint64_t)b.total;
+    *n_count = buf.missing_bases();
+    *reads = buf.is_reads() ? 1 : 0;
+    *n_seqs = buf.nseqs();
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}

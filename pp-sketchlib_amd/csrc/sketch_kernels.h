@@ -1,0 +1,51 @@
+// Sketching kernels (sketch_kernels.hip): sequences -> bit-sliced sketches.
+#pragma once
+
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace ppsk {
+
+constexpr int SKETCH_TB = 256;  // threads per hashing workgroup
+constexpr int BBITS_SK = 14;    // bit layers written per bin
+
+// A run of n k-mer start positions of one contig: windows start at
+// seq[pos], seq[pos + 1], ..., seq[pos + n - 1] and never cross the contig.
+struct SketchUnit {
+  int64_t pos;
+  int32_t n;
+  int32_t pad;
+};
+
+// A hashing workgroup's work: units [unit0, unit1) of one sample, one k.
+struct SketchGroup {
+  int64_t unit0, unit1;
+  int32_t sample;
+  int32_t kk;
+};
+
+struct SketchArgs {
+  const char* seq;           // every sample's contigs, back to back
+  const SketchUnit* units;   // per k-mer length index: its own unit list
+  const SketchGroup* groups;
+  const int32_t* klist;      // [n_k]
+  int32_t n_k;
+  int32_t s64;
+  int32_t use_rc;
+  int64_t n_samples;
+  uint64_t binsize;          // ceil((2^61 - 1) / (64 s64))
+  uint64_t bin_recip;        // floor((2^64 - 1) / binsize)
+  uint64_t* signs;           // [n_samples][n_k][64 s64] bin minima (work)
+  uint64_t* raw;             // [n_samples][n_k][s64 * 14] output words
+  double* inv_minhash;       // [n_samples][n_k] bin 0 / (2^61 - 1), before densifying
+  int32_t* densified;        // [n_samples][n_k] 0, 1, or -1 (every bin empty)
+};
+
+// Clears the bin minima, hashes every group, densifies and bit-slices.
+// lds_bins: keep each workgroup's bin minima in LDS (64 s64 * 8 bytes of
+// dynamic LDS); otherwise update the global minima directly.
+hipError_t launch_sketch(const SketchArgs& a, int64_t n_groups, bool lds_bins,
+                         hipStream_t stream);
+
+}  // namespace ppsk

@@ -32,7 +32,7 @@ EXPORTS = [
     "ppsk_last_timing", "ppsk_synth_sketches", "ppsk_set_prefix", "ppsk_batch_create",
     "ppsk_batch_run", "ppsk_batch_tuning", "ppsk_batch_free", "ppsk_server_create",
     "ppsk_server_submit", "ppsk_server_wait", "ppsk_result_free", "ppsk_server_stats",
-    "ppsk_server_free",
+    "ppsk_server_free", "ppsk_sketch",
 ]
 
 
@@ -114,6 +114,8 @@ def lib():
                                     _c.POINTER(_i32), _c.POINTER(_c.c_size_t),
                                     _c.POINTER(_c.c_double)]
     L.ppsk_server_free.argtypes = [_p]
+    L.ppsk_sketch.argtypes = [_c.c_int, _i64, _c.c_char_p, _i64, _p, _i64, _p, _i32, _p, _i32,
+                              _i32, _p, _p, _p]
     _lib = L
     return L
 
@@ -298,6 +300,31 @@ class Batch:
 
     def __exit__(self, *a):
         self.free()
+
+
+def sketch(samples, klist, sketchsize64, use_rc=True, device=0):
+    """Sketch sequences on the GPU.  samples: list of samples, each a list of
+    contigs (str or bytes).  Returns (raw [n][n_k][s64*14] uint64,
+    inv_minhash_sum [n], densified [n])."""
+    contigs = []
+    sample_contig = [0]
+    for smp in samples:
+        for c in smp:
+            contigs.append(c.encode() if isinstance(c, str) else bytes(c))
+        sample_contig.append(len(contigs))
+    seq = b"".join(contigs)
+    off = np.zeros(len(contigs) + 1, np.int64)
+    off[1:] = np.cumsum([len(c) for c in contigs])
+    sc = np.asarray(sample_contig, np.int64)
+    kl = np.ascontiguousarray(klist, np.int32)
+    n = len(samples)
+    raw = np.zeros((n, len(kl), sketchsize64 * 14), np.uint64)
+    mh = np.zeros(n, np.float64)
+    dn = np.zeros(n, np.int32)
+    _check(lib().ppsk_sketch(device, n, seq, len(seq), _ptr(off), len(contigs), _ptr(sc),
+                             len(kl), _ptr(kl), sketchsize64, int(bool(use_rc)), _ptr(raw),
+                             _ptr(mh), _ptr(dn)))
+    return raw, mh, dn
 
 
 def _copy_out(ptr, n, dtype):
