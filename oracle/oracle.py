@@ -286,6 +286,11 @@ def ref_sketch_lib():
         L.ref_read_stats.argtypes = [_c.c_char_p, _c.c_int, _p, _c.POINTER(_c.c_uint64),
                                      _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_int),
                                      _c.POINTER(_c.c_uint64)]
+        L.ref_read_seqs.restype = _c.c_int64
+        L.ref_read_seqs.argtypes = [_c.c_char_p, _c.c_int, _c.c_char_p, _c.c_uint64, _p,
+                                    _c.c_uint64]
+        L.ref_sketch_file.argtypes = [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _p,
+                                      _c.POINTER(_c.c_double), _c.POINTER(_c.c_int)]
         _REF_SKETCH = L
     return _REF_SKETCH
 
@@ -313,6 +318,37 @@ def ref_read_stats(path, kmer_len):
                         _c.byref(rd), _c.byref(ns)):
         raise RuntimeError("reference reader failed")
     return b, tot.value, nc.value, bool(rd.value), ns.value
+
+
+def ref_read_seqs(path, kmer_len, cap=1 << 26, max_seqs=1 << 20):
+    """The sequences the reference's reader keeps from one file, as str."""
+    L = ref_sketch_lib()
+    buf = _c.create_string_buffer(cap)
+    lens = np.zeros(max_seqs, np.uint64)
+    n = L.ref_read_seqs(path.encode(), int(kmer_len), buf, cap, _ptr(lens), max_seqs)
+    if n < 0:
+        raise RuntimeError("reference reader failed")
+    out, pos, raw = [], 0, buf.raw
+    for ln in lens[:n].tolist():
+        out.append(raw[pos:pos + ln].decode("latin-1"))
+        pos += ln
+    return out
+
+
+def ref_sketch_file(path, keep_len, k, sketchsize64, use_rc=True):
+    """The reference's sketch of one file at one k (sequences kept if at
+    least keep_len long); None when it keeps no sequence."""
+    L = ref_sketch_lib()
+    us = np.zeros(sketchsize64 * 14, np.uint64)
+    mh = _c.c_double()
+    dn = _c.c_int()
+    rc = L.ref_sketch_file(path.encode(), int(keep_len), int(k), int(sketchsize64),
+                           int(bool(use_rc)), _ptr(us), _c.byref(mh), _c.byref(dn))
+    if rc < 0:
+        raise RuntimeError("reference sketch failed")
+    if rc == 1:
+        return None
+    return us, mh.value, dn.value
 
 
 def knn_rows(rows, row_ids, kNN, num_threads=None):

@@ -55,3 +55,48 @@ extern "C" int ref_read_stats(const char *path, int kmer_len, double *bases4, ui
     return -1;
   }
 }
+
+// The sequences the reference's reader keeps from one file (upper-cased,
+// at least kmer_len long), back to back in out[cap]; lens[i] per sequence.
+// Returns the number of sequences, or -1 if the buffers are too small or
+// the reference threw.
+extern "C" int64_t ref_read_seqs(const char *path, int kmer_len, char *out, uint64_t cap,
+                                 uint64_t *lens, uint64_t lens_cap) {
+  try {
+    SeqBuf buf(std::vector<std::string>{path}, (size_t)kmer_len);
+    uint64_t used = 0, n = 0;
+    // an empty buffer's iterator is unset (seqio.cpp reset): stop before it
+    while (buf.nseqs() > 0 && !buf.eof()) {
+      const std::string &s = *buf.getseq();
+      if (n >= lens_cap || used + s.size() > cap) return -1;
+      std::memcpy(out + used, s.data(), s.size());
+      used += s.size();
+      lens[n++] = s.size();
+      buf.move_next_seq();
+    }
+    return (int64_t)n;
+  } catch (...) {
+    return -1;
+  }
+}
+
+// The reference's Reference sketch of one file at one k-mer length
+// (SeqBuf from the file keeping sequences at least keep_len long, then
+// sketch() with min_count 0, as constructDatabase runs it by default).
+// Returns 1 when the file holds no kept sequence (Reference throws there).
+extern "C" int ref_sketch_file(const char *path, int keep_len, int k, int s64, int use_rc,
+                               uint64_t *usigs, double *inv_minhash, int *densified) {
+  try {
+    SeqBuf buf(std::vector<std::string>{path}, (size_t)keep_len);
+    if (buf.nseqs() == 0) return 1;
+    KmerSeeds seeds = generate_seeds({(size_t)k}, false);
+    auto res = sketch(buf, (uint64_t)s64, seeds[(size_t)k], 14, false, use_rc != 0, 0, false);
+    const std::vector<uint64_t> &u = std::get<0>(res);
+    std::memcpy(usigs, u.data(), u.size() * sizeof(uint64_t));
+    *inv_minhash = std::get<1>(res);
+    *densified = std::get<2>(res) ? 1 : 0;
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}

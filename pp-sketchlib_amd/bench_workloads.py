@@ -1,7 +1,9 @@
-"""Secondary benchmark workloads of bench.py (--workload large | batch | serve).
+"""Secondary benchmark workloads of bench.py (--workload large | batch |
+serve | genomes).
 
-Each returns one JSON-ready dict.  All inputs are seeded synthetic sketches
-from the library's tree generator, resident in HBM before timing starts.
+Each returns one JSON-ready dict.  Inputs of the distance workloads are
+seeded synthetic sketches from the library's tree generator, resident in HBM
+before timing starts; the genomes workload starts from sequence files.
 
 large   32,768 samples, 8 k-mer lengths, 10,000 bins: one self pass over
         536,854,528 pairs, dense core/accessory streamed to page-locked host
@@ -19,6 +21,9 @@ serve   a resident database (default 200,000 samples, 8 k, 10,000 bins,
         Saturation throughput (every request queued at once), then Poisson
         arrivals at a fixed rate for latency percentiles, and the plain path
         (each request alone: one query-vs-ref ppsk_dense into device memory).
+genomes 64 synthetic 2 Mb genomes down a random tree, written as FASTA:
+        constructDatabase (read, sketch on the GPU at 8 k-mer lengths and
+        10,000 bins, write HDF5) and queryDatabase over the 2,016 pairs.
 """
 import json
 import math
@@ -272,3 +277,117 @@ def serve(a):
                    "kNN": 10, "threshold": 0.05, "slots": 4, "max_rows_per_step": 1024,
                    "random_correction": "equal-base"},
     }
+
+
+# ---------------------------------------------------------------------------
+def _tree_genomes(n_leaves, length, rng, n_blocks=400, block=1000):
+    """Leaves of a random binary tree over one i.i.d. ACGT ancestor (codes
+    0-3).  Each branch applies SNPs at a rate drawn from [0.02%, 0.8%], so a
+    leaf lies 0.1-5% from the root; the accessory pool holds n_blocks 1 kb
+    blocks, each branch flipping each block's presence with probability
+    [0, 5%], so leaves carry 0-20% accessory sequence."""
+    anc = rng.integers(0, 4, size=length, dtype=np.uint8)
+    pool = rng.integers(0, 4, size=(n_blocks, block), dtype=np.uint8)
+    leaves = []
+
+    def grow(core, present, k):
+        if k == 1:
+            leaves.append(np.concatenate([core, pool[present].reshape(-1)]))
+            return
+        left = int(rng.integers(1, k))
+        for part in (left, k - left):
+            c = core.copy()
+            m = rng.random(length) < rng.uniform(0.0002, 0.008)
+            c[m] = (c[m] + rng.integers(1, 4, size=int(m.sum()), dtype=np.uint8)) % 4
+            p = present ^ (rng.random(n_blocks) < rng.uniform(0.0, 0.05))
+            grow(c, p, part)
+
+    grow(anc, rng.random(n_blocks) < 0.5, n_leaves)
+    return leaves
+
+
+def genomes(a):
+    """64 synthetic 2 Mb genomes -> FASTA -> constructDatabase (GPU sketching)
+    -> queryDatabase all-vs-all (2,016 pairs)."""
+    import shutil
+    import tempfile
+
+    import pp_sketchlib as pp
+
+    n, length = 64, 2_000_000
+    rng = np.random.default_rng(17)
+    t0 = time.time()
+    g = _tree_genomes(n, length, rng)
+    work = tempfile.mkdtemp(prefix="ppsk_genomes_")
+    try:
+        letters = np.frombuffer(b"ACGT", np.uint8)
+        files, names = [], []
+        for i, codes in enumerate(g):
+            p = os.path.join(work, f"g{i}.fa")
+            with open(p, "wb") as f:
+                f.write(b">contig1\n" + letters[codes].tobytes() + b"\n")
+            files.append([p])
+            names.append(f"genome_{i}")
+        bases = int(sum(x.size for x in g))
+        print(f"# wrote {n} genomes ({bases / 1e6:.1f} Mb) in {time.time() - t0:.1f}s", flush=True)
+        db = os.path.join(work, "db")
+        walls, kms, qwalls = [], [], []
+        for rep in range(max(2, a.steps // 5 + 1)):
+            if os.path.exists(db + ".h5"):
+                os.remove(db + ".h5")
+            t = time.perf_counter()
+            pp.constructDatabase(db, names, files, KL8, S64, calc_random=False, num_threads=16)
+            walls.append(time.perf_counter() - t)
+            kms.append(H.last_timing()[0])
+            t = time.perf_counter()
+            d = pp.queryDatabase(db, db, names, names, KL8)
+            qwalls.append(time.perf_counter() - t)
+            print(f"# rep {rep}: construct {walls[-1]:.2f}s (sketch kernel {kms[-1]:.1f} ms), "
+                  f"query {qwalls[-1] * 1e3:.1f} ms", flush=True)
+        assert d.shape == (n * (n - 1) // 2, 2)
+        kmers = sum((x.size - k + 1) for x in g for k in KL8)
+        wall, ms, qwall = min(walls[1:]), min(kms[1:]), min(qwalls[1:])
+        line = {
+            "metric": "genomes sketched/s (8 k, 10,000 bins) via constructDatabase, "
+                      "then all-vs-all queryDatabase",
+            "value": round(n / wall, 2), "unit": "genomes/s", "n_gpus": 1,
+            "construct_wall_s": round(wall, 3),
+            "sketch_kernel_ms": round(ms, 2),
+            "kmer_hashes_per_s": round(kmers / (ms * 1e-3), 1),
+            "query_wall_ms": round(qwall * 1e3, 2),
+            "end_to_end_s": round(wall + qwall, 3),
+            "config": {"workload": "64 tree genomes of 2 Mb (+0-20% accessory), FASTA files",
+                       "bases": bases, "kmers": KL8, "bins": S64 * 64, "pairs": d.shape[0]},
+        }
+        if not a.no_cpu:
+            line["cpu_baseline"] = _genomes_cpu_baseline(g, names, db, d, kmers)
+        return line
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def _genomes_cpu_baseline(g, names, db, d, n_kmers):
+    """The oracle's sketch (one thread) of the first genome at every k, scaled
+    to all 64; its all-vs-all on the stored sketches (16 threads)."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "oracle"))
+    import oracle as O
+    import pp_sketchlib as pp
+
+    seq0 = np.frombuffer(b"ACGT", np.uint8)[g[0]].tobytes().decode()
+    t = time.perf_counter()
+    for k in KL8:
+        words, _, _ = O.sketch([seq0], k, S64, True)
+        assert np.array_equal(words, pp.getSketch(db, names[0])["sketch"][k])
+    dt = time.perf_counter() - t
+    per_genome = dt * (n_kmers / len(g)) / sum(g[0].size - k + 1 for k in KL8)
+    raw = np.array([[pp.getSketch(db, nm)["sketch"][k] for k in KL8] for nm in names], np.uint64)
+    lens = np.array([pp.getSketch(db, nm)["length"] for nm in names], np.uint64)
+    t = time.perf_counter()
+    want, _, _, _ = O.query_db(raw, KL8, S64, 14, lens, random=O.Random(1, True), num_threads=16)
+    dq = time.perf_counter() - t
+    return {"value": round(1.0 / per_genome, 3), "unit": "genomes/s", "cores": 1, "kind": "port",
+            "sample": f"genome 0 sketched at 8 k ({dt:.1f} s, words equal to the GPU's); "
+                      f"query of 2,016 pairs on 16 threads {dq * 1e3:.1f} ms",
+            "query_max_abs_diff": float(np.abs(d - want).max())}

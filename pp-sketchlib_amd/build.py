@@ -136,14 +136,14 @@ def build_module():
     tmp = out + ".tmp"
     py_inc = sysconfig.get_paths()["include"]
     srcs = [os.path.join(CSRC, "host", f) for f in
-            ("sketch_db.cpp", "random_match.cpp", "api.cpp", "bindings.cpp")]
+            ("sketch_db.cpp", "random_match.cpp", "api.cpp", "sketch_build.cpp", "bindings.cpp")]
     cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
            "-ffp-contract=off", "-fvisibility=hidden",
            "-I", INCLUDE, "-I", CSRC, "-I", pybind11.get_include(), "-I", py_inc,
            "-I", os.path.join(HDF5_PREFIX, "include"),
            "-DPPSK_VERSION_INFO=\"2.1.5\""] + srcs + [
            "-L", HERE, "-lppsk",
-           os.path.join(_vendor_hdf5(), "libhdf5.so.103"),
+           os.path.join(_vendor_hdf5(), "libhdf5.so.103"), "-lz",
            "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/_hdf5",
            "-o", tmp]
     _run(cmd)

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
+#include <array>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -64,6 +65,29 @@ sparse_coo sparsify_dists_by_threshold(const float* dense, size_t rows, size_t c
                                        float cutoff, unsigned num_threads);
 
 // Synthetic database writer: seeded sketches in the database layout.
+// sketch_version attribute written to new databases: the reference's
+// SKETCH_VERSION for 2.1.5, so databases compare equal across the two builds.
+constexpr const char* SKETCH_VERSION = "445afa87649c32acbf9749c77231b5b0fcb08179";
+
+// constructDatabase (src/sketchlib_bindings.cpp:45-84): sketches on the GPU.
+// What the sequence reader keeps of a sample's files (src/sketch/seqio.cpp:
+// 55-111): contigs at least kmer_len long, upper-cased; composition of the
+// first 999,999 bases; N count; whether any record had a quality string.
+struct SequenceStats {
+  std::array<double, 4> bases{};
+  size_t total = 0;
+  unsigned long missing_bases = 0;
+  bool reads = false;
+  std::vector<std::string> contigs;
+};
+SequenceStats read_sequence_stats(const std::vector<std::string>& files, size_t kmer_len);
+
+void construct_database(const std::string& db_name, const std::vector<std::string>& names,
+                        const std::vector<std::vector<std::string>>& files,
+                        std::vector<size_t> kmer_lengths, size_t sketchsize64, bool codon_phased,
+                        bool calc_random, bool use_rc, size_t min_count, size_t num_threads,
+                        int device_id);
+
 void write_synthetic_db(const std::string& db_name, int64_t n_samples,
                         const std::vector<size_t>& kmer_lengths, size_t sketchsize64,
                         uint64_t seed, double dcore_max, double dacc_max, double length_mean,

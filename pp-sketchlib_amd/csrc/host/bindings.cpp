@@ -21,7 +21,6 @@ using namespace ppsk_host;
 #endif
 // Sketch-format identifier of pp-sketchlib 2.1.5 (sha1 of its sketcher
 // sources, src/Makefile:103), so databases from that version compare equal.
-#define PPSK_SKETCH_VERSION "445afa87649c32acbf9749c77231b5b0fcb08179"
 
 namespace {
 
@@ -156,12 +155,16 @@ py::tuple sparsifyDists(const farray&, float, int, size_t) {
       "Downgrade to v2.0.0, or update the calling package");
 }
 
-void constructDatabase(const std::string&, const std::vector<std::string>&,
-                       const std::vector<std::vector<std::string>>&, std::vector<size_t>, size_t,
-                       bool, bool, bool, size_t, bool, size_t, bool, int) {
-  throw std::runtime_error(
-      "constructDatabase: sequence sketching is not part of this library, which provides the "
-      "distance path; sketch with pp-sketchlib or use writeSyntheticDatabase");
+// Sketches on the GPU whatever use_gpu says (there is no CPU path); exact
+// read counting is not offered (exact only matters with min_count > 0).
+void constructDatabase(const std::string& db_name, const std::vector<std::string>& names,
+                       const std::vector<std::vector<std::string>>& files,
+                       std::vector<size_t> kmer_lengths, size_t sketch_size, bool codon_phased,
+                       bool calc_random, bool use_rc, size_t min_count, bool /*exact*/,
+                       size_t num_threads, bool /*use_gpu*/, int device_id) {
+  py::gil_scoped_release release;
+  ppsk_host::construct_database(db_name, names, files, kmer_lengths, sketch_size, codon_phased,
+                                calc_random, use_rc, min_count, num_threads, device_id);
 }
 
 void addRandom(const std::string&, const std::vector<std::string>&, std::vector<size_t>, bool,
@@ -212,6 +215,30 @@ void writeRandomMatches(const std::string& db_name, const std::vector<std::strin
 
 std::vector<std::string> getSampleNames(const std::string& db_name) {
   return list_sketch_names(db_name + ".h5");
+}
+
+// One stored sample: attributes and the words of every k-mer length, as the
+// reference's Reference accessors expose them (src/reference.hpp:60-79).
+py::dict getSketch(const std::string& db_name, const std::string& name) {
+  auto v = load_sketches(db_name, {name}, {}, false);
+  if (v.empty()) throw std::runtime_error("sample " + name + " not found in " + db_name);
+  const Sketch& s = v[0];
+  py::dict d, words;
+  for (size_t i = 0; i < s.kmers.size(); ++i) {
+    py::array_t<uint64_t> a(s.usigs[i].size());
+    std::memcpy(a.mutable_data(), s.usigs[i].data(), s.usigs[i].size() * sizeof(uint64_t));
+    words[py::int_(s.kmers[i])] = a;
+  }
+  d["name"] = s.name;
+  d["kmers"] = s.kmers;
+  d["sketchsize64"] = s.sketchsize64;
+  d["bbits"] = s.bbits;
+  d["length"] = s.seq_length;
+  d["missing_bases"] = s.missing_bases;
+  d["reads"] = s.reads;
+  d["base_freq"] = std::vector<double>(s.bases.begin(), s.bases.end());
+  d["sketch"] = words;
+  return d;
 }
 
 }  // namespace
@@ -272,6 +299,17 @@ PYBIND11_MODULE(pp_sketchlib, m) {
         py::arg("length_mean") = 2000000.0, py::arg("name_prefix") = "sample");
   m.def("getSampleNames", &getSampleNames, "Sample names of a database, in HDF5 name order",
         py::arg("db_name"));
+  m.def("getSketch", &getSketch, "Attributes and sketch words of one stored sample",
+        py::arg("db_name"), py::arg("name"));
+  m.def(
+      "_readSequences",
+      [](const std::vector<std::string>& files, size_t kmer_len) {
+        auto st = ppsk_host::read_sequence_stats(files, kmer_len);
+        return py::make_tuple(std::vector<double>(st.bases.begin(), st.bases.end()), st.total,
+                              st.missing_bases, st.reads, st.contigs);
+      },
+      "What the sequence reader keeps of one sample's files", py::arg("files"),
+      py::arg("kmer_len"));
   m.def("writeRandomMatches", &writeRandomMatches,
         "Store Monte Carlo random-match tables in a database's /random group",
         py::arg("db_name"), py::arg("table_keys"), py::arg("table_values"),
@@ -279,7 +317,7 @@ PYBIND11_MODULE(pp_sketchlib, m) {
         py::arg("k_max"), py::arg("use_rc") = true);
 
   m.attr("version") = PPSK_VERSION_INFO;
-  m.attr("sketchVersion") = PPSK_SKETCH_VERSION;
+  m.attr("sketchVersion") = ppsk_host::SKETCH_VERSION;
 
   py::register_exception<HDF5Error>(m, "HDF5Exception");
 }

@@ -1,0 +1,263 @@
+// constructDatabase: read sequence files, sketch on the GPU (ppsk_sketch),
+// write the reference's HDF5 layout.
+//
+// Reading follows the reference's SeqBuf (src/sketch/seqio.cpp:55-111) over
+// kseq's record rules (src/sketch/kseq.h, __KSEQ_READ):
+//  * a record starts at '>' or '@'; its sequence is every following line up
+//    to a line starting with '>', '@' or '+' (empty lines skipped, a trailing
+//    '\r' dropped);
+//  * after '+', quality lines are read until they are as long as the
+//    sequence; a length mismatch ends the file (kseq_read returns -2);
+//  * records shorter than the largest k-mer length are dropped; kept ones are
+//    upper-cased; the composition counts A, C, G, T/U among the first
+//    999,999 bases (the counter is tested after its increment), divided by
+//    min(total, 1,000,000); N is counted over all kept bases;
+//  * any quality string marks the sample as reads.
+// Sketching options follow create_sketches (src/api.cpp:39-138): an existing
+// database holding every requested sample is loaded instead of re-sketched.
+#include <zlib.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "api.hpp"
+#include "sketch_db.hpp"
+#include "../../../include/ppsk.h"
+
+namespace ppsk_host {
+
+namespace {
+
+constexpr size_t MAX_COMPOSITION_SAMPLE = 1000000;  // seqio.cpp:24
+
+struct SampleSeq {
+  std::vector<std::string> contigs;
+  size_t total = 0, a = 0, c = 0, g = 0, t = 0;
+  unsigned long n_count = 0;
+  bool reads = false;
+};
+
+std::string read_all(const std::string& path) {
+  gzFile fp = gzopen(path.c_str(), "r");
+  if (!fp) throw std::runtime_error("Could not open " + path);
+  std::string buf;
+  std::vector<char> chunk(1 << 20);
+  int n;
+  while ((n = gzread(fp, chunk.data(), (unsigned)chunk.size())) > 0) buf.append(chunk.data(), n);
+  gzclose(fp);
+  if (n < 0) throw std::runtime_error("Could not read " + path);
+  return buf;
+}
+
+// Appends the line starting at i (without its '\n') to `to`, dropping a
+// trailing '\r' when `to` is then longer than one character (kseq's test is
+// on the accumulated string); i moves past the line.
+void append_line(const std::string& b, size_t& i, std::string& to) {
+  const size_t e = b.find('\n', i);
+  const size_t end = e == std::string::npos ? b.size() : e;
+  to.append(b, i, end - i);
+  if (to.size() > 1 && to.back() == '\r') to.pop_back();
+  i = e == std::string::npos ? b.size() : e + 1;
+}
+
+void read_file(const std::string& path, size_t kmer_len, SampleSeq& out) {
+  const std::string b = read_all(path);
+  size_t i = 0;
+  std::string skip;
+  for (;;) {
+    // to the next header character (anywhere, as kseq after a quality string)
+    while (i < b.size() && b[i] != '>' && b[i] != '@') ++i;
+    if (i >= b.size()) break;
+    ++i;                         // header character
+    skip.clear();
+    append_line(b, i, skip);     // name and comment
+    std::string seq;
+    while (i < b.size() && b[i] != '>' && b[i] != '+' && b[i] != '@') {
+      if (b[i] == '\n') {
+        ++i;
+        continue;
+      }
+      append_line(b, i, seq);
+    }
+    bool qual = false;
+    if (i < b.size() && b[i] == '+') {
+      skip.clear();
+      append_line(b, i, skip);   // '+' line
+      // kseq reads at least one quality line, then more while it is short
+      std::string q;
+      do {
+        if (i >= b.size()) break;
+        append_line(b, i, q);
+      } while (q.size() < seq.size());
+      if (q.size() != seq.size()) return;  // kseq: a quality of another length ends the file
+      qual = !q.empty();
+    }
+    seq.resize(std::strlen(seq.c_str()));  // the reader keeps the C string (seqio.cpp:68-73)
+    const size_t seq_len = seq.size();
+    if (seq_len >= kmer_len) {
+      for (char& ch : seq) {
+        if ('a' <= ch && ch <= 'z') ch = (char)(ch ^ 0x20);
+        out.total++;
+        if (out.total < MAX_COMPOSITION_SAMPLE) {
+          switch (ch) {
+            case 'A':
+              out.a++;
+              break;
+            case 'C':
+              out.c++;
+              break;
+            case 'G':
+              out.g++;
+              break;
+            case 'T':
+            case 'U':
+              out.t++;
+              break;
+          }
+        }
+        if (ch == 'N') out.n_count++;
+      }
+      out.contigs.push_back(std::move(seq));
+    }
+    if (qual) out.reads = true;
+  }
+}
+
+}  // namespace
+
+SequenceStats read_sequence_stats(const std::vector<std::string>& files, size_t kmer_len) {
+  SampleSeq q;
+  for (const auto& f : files) read_file(f, kmer_len, q);
+  SequenceStats st;
+  const double tot = (double)std::min(q.total, MAX_COMPOSITION_SAMPLE);
+  st.bases = {q.a / tot, q.c / tot, q.g / tot, q.t / tot};
+  st.total = q.total;
+  st.missing_bases = q.n_count;
+  st.reads = q.reads;
+  st.contigs = std::move(q.contigs);
+  return st;
+}
+
+namespace {
+
+bool file_exists(const std::string& p) {
+  struct stat st;
+  return stat(p.c_str(), &st) == 0;
+}
+
+}  // namespace
+
+void construct_database(const std::string& db_name, const std::vector<std::string>& names,
+                        const std::vector<std::vector<std::string>>& files,
+                        std::vector<size_t> kmer_lengths, size_t sketchsize64, bool codon_phased,
+                        bool calc_random, bool use_rc, size_t min_count, size_t num_threads,
+                        int device_id) {
+  if (codon_phased) {
+    throw std::runtime_error("Codon phased seeds not yet implemented for GPU sketching");
+  }
+  if (names.size() != files.size()) throw std::runtime_error("names and files differ in length");
+  std::sort(kmer_lengths.begin(), kmer_lengths.end());
+  if (kmer_lengths.empty() || kmer_lengths.front() < 3) {
+    throw std::runtime_error("Minimum k must be 3 or higher");
+  }
+  const std::string h5 = db_name + ".h5";
+  if (file_exists(h5)) {
+    // create_sketches reuses a database that already holds every sample
+    if (load_sketches(db_name, names, kmer_lengths, false).size() == names.size()) return;
+  }
+  std::cerr << "Sketching " << names.size() << " genomes on GPU device " << device_id << std::endl;
+  std::vector<SampleSeq> seqs(names.size());
+  std::vector<std::string> errors(names.size());
+#pragma omp parallel for schedule(dynamic, 1) num_threads(std::max<size_t>(num_threads, 1))
+  for (long i = 0; i < (long)names.size(); ++i) {
+    try {
+      for (const auto& f : files[i]) read_file(f, kmer_lengths.back(), seqs[i]);
+      if (seqs[i].contigs.empty()) errors[i] = names[i] + " contains no sequence";
+      if (seqs[i].reads && min_count > 0) {
+        errors[i] = "read filtering (min_count > 0) is not supported by this build: " + names[i];
+      }
+    } catch (const std::exception& e) {
+      errors[i] = e.what();
+    }
+  }
+  // as create_sketches (src/api.cpp:100-112): messages on stdout, then throw
+  bool any_error = false;
+  for (const auto& e : errors) {
+    if (!e.empty()) {
+      std::cout << e << std::endl;
+      any_error = true;
+    }
+  }
+  if (any_error) throw std::runtime_error("Errors during sketching");
+
+  std::vector<int32_t> kl(kmer_lengths.begin(), kmer_lengths.end());
+  const size_t K = kl.size(), words = sketchsize64 * 14;
+  write_new_db(h5, use_rc, false, SKETCH_VERSION);
+  // batches of samples bounded by their bases (device copy of the sequence)
+  const size_t batch_bases = (size_t)1 << 31;
+  size_t s0 = 0;
+  while (s0 < seqs.size()) {
+    size_t s1 = s0, bases = 0;
+    while (s1 < seqs.size() && (s1 == s0 || bases + seqs[s1].total <= batch_bases)) {
+      bases += seqs[s1].total;
+      ++s1;
+    }
+    std::string seq;
+    seq.reserve(bases);
+    std::vector<int64_t> off{0}, sc{0};
+    for (size_t s = s0; s < s1; ++s) {
+      for (const auto& c : seqs[s].contigs) {
+        seq += c;
+        off.push_back((int64_t)seq.size());
+      }
+      sc.push_back((int64_t)off.size() - 1);
+    }
+    const int64_t n = (int64_t)(s1 - s0);
+    std::vector<uint64_t> raw((size_t)n * K * words);
+    std::vector<double> mh(n);
+    std::vector<int32_t> dn(n);
+    const int rc = ppsk_sketch(device_id, n, seq.data(), (int64_t)seq.size(), off.data(),
+                               (int64_t)off.size() - 1, sc.data(), (int32_t)K, kl.data(),
+                               (int32_t)sketchsize64, use_rc ? 1 : 0, raw.data(), mh.data(),
+                               dn.data());
+    if (rc != PPSK_OK) throw std::runtime_error(ppsk_last_error());
+    std::vector<Sketch> out;
+    for (int64_t x = 0; x < n; ++x) {
+      const SampleSeq& q = seqs[s0 + x];
+      Sketch s;
+      s.name = names[s0 + x];
+      s.sketchsize64 = sketchsize64;
+      s.bbits = 14;
+      s.reads = q.reads;
+      s.use_rc = use_rc;
+      s.missing_bases = q.n_count;
+      const double tot = (double)std::min(q.total, MAX_COMPOSITION_SAMPLE);
+      s.bases = {q.a / tot, q.c / tot, q.g / tot, q.t / tot};
+      // reads: size from the mean bin-0 minhash over k (reference.cpp:49-55)
+      s.seq_length = q.reads ? (size_t)((double)K / mh[x]) : q.total;
+      s.kmers = kmer_lengths;
+      for (size_t kk = 0; kk < K; ++kk) {
+        const uint64_t* src = raw.data() + ((size_t)x * K + kk) * words;
+        s.usigs.emplace_back(src, src + words);
+      }
+      if (dn[x]) std::cerr << "NOTE: " << s.name << " required densification" << std::endl;
+      out.push_back(std::move(s));
+    }
+    write_sketches(h5, out);
+    s0 = s1;
+  }
+  if (calc_random) {
+    std::cerr << "Random match chances are not calculated by this build: the database has no "
+                 "/random group and queries use the equal-base correction"
+              << std::endl;
+  }
+}
+
+}  // namespace ppsk_host
