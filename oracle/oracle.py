@@ -351,6 +351,82 @@ def ref_sketch_file(path, keep_len, k, sketchsize64, use_rc=True):
     return us, mh.value, dn.value
 
 
+# --------------------------------------------------------------------------
+# Monte Carlo random-match tables (random/random_match.cpp:62-163)
+# --------------------------------------------------------------------------
+_REF_RANDOM = None
+
+
+def ref_random_lib():
+    """The reference's random generators (oracle/_ref), or None if not built."""
+    global _REF_RANDOM
+    if _REF_RANDOM is None:
+        path = os.path.join(_HERE, "_ref", "libref_random.so")
+        if not os.path.exists(path):
+            return None
+        L = _c.CDLL(path)
+        L.ref_xoshiro.argtypes = [_c.c_uint64, _c.c_int, _c.c_uint64, _p]
+        L.ref_mt_double.argtypes = [_c.c_uint32, _c.c_uint64, _p]
+        _REF_RANDOM = L
+    return _REF_RANDOM
+
+
+def ref_xoshiro(seed, n, jumps=0):
+    out = np.zeros(n, np.uint64)
+    ref_random_lib().ref_xoshiro(seed, jumps, n, _ptr(out))
+    return out
+
+
+def ref_mt_double(seed, n):
+    out = np.zeros(n, np.float64)
+    ref_random_lib().ref_mt_double(seed, n, _ptr(out))
+    return out
+
+
+def csrs_py(k, use_rc, l1, l2):
+    """random_match.cpp:318-336 (k < 33)."""
+    m = float(1 << ((k - 1) * 2 + (1 if use_rc else 0)))
+    j1 = 1 - (1 - 1 / m) ** l1
+    j2 = 1 - (1 - 1 / m) ** l2
+    return (j1 * j2) / (j1 + j2 - j1 * j2) if (j1 > 0 and j2 > 0) else 0.0
+
+
+def mc_kmer_lengths(length, sketchsize64, use_rc):
+    """random_match.cpp:82-95: k = 3, 4, ... while the equal-base chance for
+    two genomes of sketch 0's length exceeds 1 / (64 s64)."""
+    kl = [3]
+    for k in range(4, 32):
+        if csrs_py(k, use_rc, length, length) > 1.0 / (sketchsize64 * 64):
+            kl.append(k)
+        else:
+            break
+    return kl
+
+
+def mc_matches(genomes, klist, sketchsize64, n_clusters, n_mc, use_rc=True):
+    """random_match.cpp:120-156: per k a C x C float table, the mean
+    uncorrected Jaccard over genome pairs of two clusters (genomes in
+    cluster-major order), a cluster's own entry without its n_mc self pairs.
+    Sums run copy-major in double, self pairs counting 1 and removed after."""
+    sk = [[sketch([g], k, sketchsize64, use_rc)[0] for k in klist] for g in genomes]
+    out = np.zeros((len(klist), n_clusters, n_clusters), np.float32)
+    for kk in range(len(klist)):
+        for ci in range(n_clusters):
+            for cj in range(ci, n_clusters):
+                total = 0.0
+                for a in range(n_mc):
+                    for b in range(n_mc):
+                        x, y = ci * n_mc + a, cj * n_mc + b
+                        c = calc_intersize(sk[x][kk], sk[y][kk], sketchsize64, 14)
+                        total += c / float(64 * sketchsize64)
+                count = n_mc * n_mc
+                if ci == cj:
+                    total -= n_mc
+                    count -= n_mc
+                out[kk, ci, cj] = out[kk, cj, ci] = np.float32(total / count)
+    return out
+
+
 def knn_rows(rows, row_ids, kNN, num_threads=None):
     """The same heap over dense rows [m][n]; column row_ids[r] counts as +inf
     (row_ids[r] < 0: none excluded).  Returns (i_vec, j_vec, dists)."""

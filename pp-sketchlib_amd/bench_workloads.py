@@ -345,6 +345,16 @@ def genomes(a):
             print(f"# rep {rep}: construct {walls[-1]:.2f}s (sketch kernel {kms[-1]:.1f} ms), "
                   f"query {qwalls[-1] * 1e3:.1f} ms", flush=True)
         assert d.shape == (n * (n - 1) // 2, 2)
+        # Monte Carlo random-match tables on the last database, then the
+        # query again with them
+        t = time.perf_counter()
+        pp.addRandom(db, names, KL8, True, 16)
+        t_random = time.perf_counter() - t
+        t = time.perf_counter()
+        d_mc = pp.queryDatabase(db, db, names, names, KL8)
+        t_query_mc = time.perf_counter() - t
+        print(f"# random tables {t_random:.2f}s, query with them {t_query_mc * 1e3:.1f} ms",
+              flush=True)
         kmers = sum((x.size - k + 1) for x in g for k in KL8)
         wall, ms, qwall = min(walls[1:]), min(kms[1:]), min(qwalls[1:])
         line = {
@@ -356,6 +366,9 @@ def genomes(a):
             "kmer_hashes_per_s": round(kmers / (ms * 1e-3), 1),
             "query_wall_ms": round(qwall * 1e3, 2),
             "end_to_end_s": round(wall + qwall, 3),
+            "random_tables_s": round(t_random, 3),
+            "query_mc_wall_ms": round(t_query_mc * 1e3, 2),
+            "query_mc_finite": bool(np.isfinite(d_mc).all()),
             "config": {"workload": "64 tree genomes of 2 Mb (+0-20% accessory), FASTA files",
                        "bases": bases, "kmers": KL8, "bins": S64 * 64, "pairs": d.shape[0]},
         }

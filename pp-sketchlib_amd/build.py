@@ -136,7 +136,8 @@ def build_module():
     tmp = out + ".tmp"
     py_inc = sysconfig.get_paths()["include"]
     srcs = [os.path.join(CSRC, "host", f) for f in
-            ("sketch_db.cpp", "random_match.cpp", "api.cpp", "sketch_build.cpp", "bindings.cpp")]
+            ("sketch_db.cpp", "random_match.cpp", "random_build.cpp", "api.cpp", "sketch_build.cpp",
+             "bindings.cpp")]
     cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
            "-ffp-contract=off", "-fvisibility=hidden",
            "-I", INCLUDE, "-I", CSRC, "-I", pybind11.get_include(), "-I", py_inc,

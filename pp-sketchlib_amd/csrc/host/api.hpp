@@ -69,6 +69,51 @@ sparse_coo sparsify_dists_by_threshold(const float* dense, size_t rows, size_t c
 // SKETCH_VERSION for 2.1.5, so databases compare equal across the two builds.
 constexpr const char* SKETCH_VERSION = "445afa87649c32acbf9749c77231b5b0fcb08179";
 
+// xoshiro256** with the reference's seeding (each state word the splitmix64
+// of the previous one, src/random/rng.cpp:18-30); a C++11 URNG.
+class Xoshiro256 {
+ public:
+  using result_type = uint64_t;
+  static constexpr size_t min() { return 0; }
+  static constexpr size_t max() { return ~(size_t)0; }
+  explicit Xoshiro256(uint64_t seed);
+  uint64_t operator()();
+  void jump();  // 2^128 steps ahead
+
+ private:
+  static uint64_t splitmix(uint64_t x);
+  uint64_t s_[4];
+};
+
+// KMeansRex with kmeans++ seeding from a 32-bit Mersenne Twister
+// (src/random/kmeans/KMeansRexCore.cpp).  x: n rows of d values.
+struct KMeansResult {
+  std::vector<double> centroids;  // k x d
+  std::vector<int> assignment;    // n
+  int iterations = 0;
+};
+KMeansResult kmeans_plusplus(const std::vector<double>& x, size_t n, size_t d, size_t k,
+                             int max_iter, int seed);
+
+// Random genomes of the Monte Carlo tables (generate_random_sequence and the
+// generator streams of the RandomMC constructor's OpenMP loop,
+// src/random/random_match.cpp:97-118, 400-422): n_mc per cluster, in
+// cluster-major order; centroids is C x 4 (A, C, G, T).
+std::vector<std::string> random_genomes(const std::vector<float>& centroids, size_t n_clusters,
+                                        unsigned n_mc, size_t length, bool use_rc,
+                                        int num_threads);
+
+// The RandomMC constructor (src/random/random_match.cpp:62-163): random
+// genomes sketched and compared on the GPU.
+RandomTables build_random(const std::vector<Sketch>& sketches, unsigned n_clusters, unsigned n_mc,
+                          bool codon_phased, bool use_rc, int num_threads, int device_id);
+// calculate_random (src/api.cpp:518-531): build_random, saved under /random.
+void calculate_random(const std::vector<Sketch>& sketches, const std::string& db_name,
+                      unsigned n_clusters, unsigned n_mc, bool codon_phased, bool use_rc,
+                      int num_threads, int device_id);
+constexpr unsigned DEFAULT_N_CLUSTERS = 3;  // src/random/random_match.hpp:20-21
+constexpr unsigned DEFAULT_N_MC = 5;
+
 // constructDatabase (src/sketchlib_bindings.cpp:45-84): sketches on the GPU.
 // What the sequence reader keeps of a sample's files (src/sketch/seqio.cpp:
 // 55-111): contigs at least kmer_len long, upper-cased; composition of the

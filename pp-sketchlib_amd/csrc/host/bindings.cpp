@@ -167,11 +167,19 @@ void constructDatabase(const std::string& db_name, const std::vector<std::string
                                 calc_random, use_rc, min_count, num_threads, device_id);
 }
 
-void addRandom(const std::string&, const std::vector<std::string>&, std::vector<size_t>, bool,
-               size_t) {
-  throw std::runtime_error(
-      "addRandom: building random-match tables is not part of this library, which provides "
-      "the distance path; databases without a /random group use the equal-base correction");
+// addRandomToDb (src/sketchlib_bindings.cpp:189-202): Monte Carlo tables from
+// the stored sketches, written under /random.
+void addRandom(const std::string& db_name, const std::vector<std::string>& names,
+               std::vector<size_t> klist, bool use_rc, size_t num_threads) {
+  const DbAttrs attrs = read_db_attrs(db_name + ".h5");
+  py::gil_scoped_release release;
+  auto sketches = load_sketches(db_name, names, klist, false);
+  if (sketches.size() != names.size()) {
+    throw std::runtime_error("addRandom: some samples were not found in " + db_name);
+  }
+  ppsk_host::calculate_random(sketches, db_name, ppsk_host::DEFAULT_N_CLUSTERS,
+                              ppsk_host::DEFAULT_N_MC, attrs.codon_phased, use_rc,
+                              (int)num_threads, 0);
 }
 
 void writeSyntheticDatabase(const std::string& db_name, int64_t n_samples,
@@ -301,6 +309,71 @@ PYBIND11_MODULE(pp_sketchlib, m) {
         py::arg("db_name"));
   m.def("getSketch", &getSketch, "Attributes and sketch words of one stored sample",
         py::arg("db_name"), py::arg("name"));
+  m.def(
+      "_xoshiro",
+      [](uint64_t seed, size_t n, int jumps) {
+        ppsk_host::Xoshiro256 g(seed);
+        for (int j = 0; j < jumps; ++j) g.jump();
+        py::array_t<uint64_t> a(n);
+        for (size_t i = 0; i < n; ++i) a.mutable_data()[i] = g();
+        return a;
+      },
+      "xoshiro256** stream of the random genome generator", py::arg("seed"), py::arg("n"),
+      py::arg("jumps") = 0);
+  m.def(
+      "_kmeans",
+      [](py::array_t<double, py::array::c_style | py::array::forcecast> x, size_t k, int seed,
+         int max_iter) {
+        if (x.ndim() != 2) throw std::runtime_error("x must be 2-d");
+        const size_t n = x.shape(0), d = x.shape(1);
+        std::vector<double> v(x.data(), x.data() + n * d);
+        auto r = ppsk_host::kmeans_plusplus(v, n, d, k, max_iter, seed);
+        py::array_t<double> c({k, d});
+        std::copy(r.centroids.begin(), r.centroids.end(), c.mutable_data());
+        return py::make_tuple(c, r.assignment, r.iterations);
+      },
+      "k-means (kmeans++ seeding, Lloyd) as the random-match clustering runs it", py::arg("x"),
+      py::arg("k"), py::arg("seed") = 3019, py::arg("max_iter") = 300);
+  m.def(
+      "_randomGenomes",
+      [](py::array_t<float, py::array::c_style | py::array::forcecast> centroids, unsigned n_mc,
+         size_t length, bool use_rc, int num_threads) {
+        if (centroids.ndim() != 2 || centroids.shape(1) != 4) {
+          throw std::runtime_error("centroids must be [C][4]");
+        }
+        std::vector<float> c(centroids.data(), centroids.data() + centroids.size());
+        std::vector<std::string> g;
+        {
+          py::gil_scoped_release release;
+          g = ppsk_host::random_genomes(c, centroids.shape(0), n_mc, length, use_rc, num_threads);
+        }
+        return g;
+      },
+      "Random genomes of the Monte Carlo tables", py::arg("centroids"), py::arg("n_mc"),
+      py::arg("length"), py::arg("use_rc"), py::arg("num_threads"));
+  m.def(
+      "_readRandom",
+      [](const std::string& db_name) {
+        auto r = read_random(db_name + ".h5");
+        py::dict d;
+        d["present"] = r.present;
+        if (!r.present) return d;
+        const size_t C = r.n_clusters;
+        py::array_t<float> m({r.matches_keys.size(), C, C});
+        std::copy(r.matches.begin(), r.matches.end(), m.mutable_data());
+        py::array_t<float> c({C, (size_t)4});
+        std::copy(r.centroids.begin(), r.centroids.end(), c.mutable_data());
+        d["use_rc"] = r.use_rc;
+        d["k_min"] = r.k_min;
+        d["k_max"] = r.k_max;
+        d["table_keys"] = r.table_keys;
+        d["table_values"] = r.table_values;
+        d["matches_keys"] = r.matches_keys;
+        d["matches"] = m;
+        d["centroids"] = c;
+        return d;
+      },
+      "The /random group of a database", py::arg("db_name"));
   m.def(
       "_readSequences",
       [](const std::vector<std::string>& files, size_t kmer_len) {

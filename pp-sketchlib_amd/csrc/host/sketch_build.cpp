@@ -154,6 +154,14 @@ bool file_exists(const std::string& p) {
 
 }  // namespace
 
+// Reads and sketches every sample, writes a new database; returns the
+// sketches as written.
+std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::string>& names,
+                                 const std::vector<std::vector<std::string>>& files,
+                                 const std::vector<size_t>& kmer_lengths, size_t sketchsize64,
+                                 bool use_rc, size_t min_count, size_t num_threads,
+                                 int device_id);
+
 void construct_database(const std::string& db_name, const std::vector<std::string>& names,
                         const std::vector<std::vector<std::string>>& files,
                         std::vector<size_t> kmer_lengths, size_t sketchsize64, bool codon_phased,
@@ -168,10 +176,33 @@ void construct_database(const std::string& db_name, const std::vector<std::strin
     throw std::runtime_error("Minimum k must be 3 or higher");
   }
   const std::string h5 = db_name + ".h5";
+  std::vector<Sketch> all;
+  bool resketch = true;
   if (file_exists(h5)) {
     // create_sketches reuses a database that already holds every sample
-    if (load_sketches(db_name, names, kmer_lengths, false).size() == names.size()) return;
+    all = load_sketches(db_name, names, kmer_lengths, false);
+    resketch = all.size() != names.size();
   }
+  if (resketch) {
+    all = sketch_files(h5, names, files, kmer_lengths, sketchsize64, use_rc, min_count,
+                       num_threads, device_id);
+  }
+  // constructDatabase (src/sketchlib_bindings.cpp:77-86)
+  if (calc_random) {
+    if (all.size() >= DEFAULT_N_CLUSTERS) {
+      calculate_random(all, db_name, DEFAULT_N_CLUSTERS, DEFAULT_N_MC, codon_phased, use_rc,
+                       (int)num_threads, device_id);
+    } else {
+      std::cerr << "Too few input genomes to calculate random match chances" << std::endl;
+    }
+  }
+}
+
+std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::string>& names,
+                                 const std::vector<std::vector<std::string>>& files,
+                                 const std::vector<size_t>& kmer_lengths, size_t sketchsize64,
+                                 bool use_rc, size_t min_count, size_t num_threads,
+                                 int device_id) {
   std::cerr << "Sketching " << names.size() << " genomes on GPU device " << device_id << std::endl;
   std::vector<SampleSeq> seqs(names.size());
   std::vector<std::string> errors(names.size());
@@ -197,6 +228,8 @@ void construct_database(const std::string& db_name, const std::vector<std::strin
   }
   if (any_error) throw std::runtime_error("Errors during sketching");
 
+  std::vector<Sketch> all;
+  all.reserve(names.size());
   std::vector<int32_t> kl(kmer_lengths.begin(), kmer_lengths.end());
   const size_t K = kl.size(), words = sketchsize64 * 14;
   write_new_db(h5, use_rc, false, SKETCH_VERSION);
@@ -251,13 +284,10 @@ void construct_database(const std::string& db_name, const std::vector<std::strin
       out.push_back(std::move(s));
     }
     write_sketches(h5, out);
+    for (auto& x : out) all.push_back(std::move(x));
     s0 = s1;
   }
-  if (calc_random) {
-    std::cerr << "Random match chances are not calculated by this build: the database has no "
-                 "/random group and queries use the equal-base correction"
-              << std::endl;
-  }
+  return all;
 }
 
 }  // namespace ppsk_host
