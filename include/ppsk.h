@@ -310,8 +310,9 @@ int ppsk_last_timing(double *kernel_ms, int64_t *n_launches, int64_t *n_items);
  * the reference layout; inv_minhash (may be NULL): per sample the sum over k
  * of bin 0 / (2^61 - 1) before densification (the reads size estimate is
  * n_k / that sum, src/reference.cpp:49-55); densified (may be NULL): per
- * sample 1 if any k needed densification.  Replaces create_sketches /
- * sketch_gpu (src/api.cpp:39-138, src/sketch/sketch.cpp:178-205).
+ * sample 1 if any k needed densification.  klist: ascending, 3 <= k <= 255;
+ * 1 <= sketchsize64 <= 1023.  Replaces create_sketches / sketch_gpu
+ * (src/api.cpp:39-138, src/sketch/sketch.cpp:178-205).
  */
 int ppsk_sketch(int device_id, int64_t n_samples, const char *seq,
                 int64_t seq_bytes, const int64_t *contig_off,

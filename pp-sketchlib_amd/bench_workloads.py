@@ -1,5 +1,5 @@
 """Secondary benchmark workloads of bench.py (--workload large | batch |
-serve | genomes).
+serve | sketch | genomes).
 
 Each returns one JSON-ready dict.  Inputs of the distance workloads are
 seeded synthetic sketches from the library's tree generator, resident in HBM
@@ -21,6 +21,8 @@ serve   a resident database (default 200,000 samples, 8 k, 10,000 bins,
         Saturation throughput (every request queued at once), then Poisson
         arrivals at a fixed rate for latency percentiles, and the plain path
         (each request alone: one query-vs-ref ppsk_dense into device memory).
+sketch  the same 64 genomes held in memory, sketched by ppsk_sketch (8 k,
+        10,000 bins): the GPU sketcher alone, inputs from host memory.
 genomes 64 synthetic 2 Mb genomes down a random tree, written as FASTA:
         constructDatabase (read, sketch on the GPU at 8 k-mer lengths and
         10,000 bins, write HDF5) and queryDatabase over the 2,016 pairs.
@@ -404,3 +406,30 @@ def _genomes_cpu_baseline(g, names, db, d, n_kmers):
             "sample": f"genome 0 sketched at 8 k ({dt:.1f} s, words equal to the GPU's); "
                       f"query of 2,016 pairs on 16 threads {dq * 1e3:.1f} ms",
             "query_max_abs_diff": float(np.abs(d - want).max())}
+
+
+def sketch(a):
+    """ppsk_sketch over 64 tree genomes of 2 Mb held in host memory."""
+    n, length = 64, 2_000_000
+    g = _tree_genomes(n, length, np.random.default_rng(17))
+    letters = np.frombuffer(b"ACGT", np.uint8)
+    samples = [[letters[x].tobytes()] for x in g]
+    kmers = sum(x.size - k + 1 for x in g for k in KL8)
+    bases = int(sum(x.size for x in g))
+    print(f"# {n} genomes, {bases / 1e6:.1f} Mb, {kmers / 1e9:.3f} G k-mers over 8 k", flush=True)
+    walls, kms = [], []
+    for rep in range(max(2, a.steps)):
+        t = time.perf_counter()
+        H.sketch(samples, KL8, S64)
+        walls.append(time.perf_counter() - t)
+        kms.append(H.last_timing()[0])
+        print(f"# rep {rep}: wall {walls[-1] * 1e3:.1f} ms, kernels {kms[-1]:.2f} ms", flush=True)
+    ms, wall = min(kms[1:]), min(walls[1:])
+    return {
+        "metric": "k-mer hashes/s, GPU sketching (8 k, 10,000 bins, canonical)",
+        "value": round(kmers / (ms * 1e-3), 1), "unit": "k-mers/s", "n_gpus": 1,
+        "kernels_ms": round(ms, 3), "wall_ms": round(wall * 1e3, 2),
+        "genomes_per_s_wall": round(n / wall, 1),
+        "config": {"workload": "64 tree genomes of 2 Mb (+0-20% accessory) in host memory",
+                   "bases": bases, "kmers": KL8, "bins": S64 * 64, "kmer_windows": kmers},
+    }

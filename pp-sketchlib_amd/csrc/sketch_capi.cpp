@@ -43,6 +43,9 @@ int ppsk_sketch(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
   }
   for (int i = 0; i < n_k; ++i) {
     if (klist[i] < 3) return fail(PPSK_ERR_ARG, "Minimum k must be 3 or higher");
+    if (klist[i] > ppsk::SKETCH_MAX_K) {
+      return fail(PPSK_ERR_ARG, "k must be at most " + std::to_string(ppsk::SKETCH_MAX_K));
+    }
     if (i && klist[i] <= klist[i - 1]) return fail(PPSK_ERR_ARG, "klist must be strictly ascending");
   }
   if (sketchsize64 < 1 || sketchsize64 > 1023) {
@@ -95,9 +98,11 @@ int ppsk_sketch(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
   a.binsize = (sign_mod + (uint64_t)nbins - 1ULL) / (uint64_t)nbins;
   a.bin_recip = ~0ULL / a.binsize;
 
-  DevBuf d_seq, d_units, d_groups, d_k, d_signs, d_raw, d_mh, d_dens;
+  DevBuf d_seq, d_units, d_groups, d_k, d_signs, d_raw, d_mh, d_dens, d_next;
   const size_t raw_words = (size_t)n_samples * n_k * sketchsize64 * ppsk::BBITS_SK;
-  hipError_t e = hipMalloc(&d_seq.p, std::max<int64_t>(seq_bytes, 1));
+  // base codes, padded both sides for the kernel's stream loads
+  hipError_t e =
+      hipMalloc(&d_seq.p, (size_t)(ppsk::CODE_PAD_FRONT + seq_bytes + ppsk::CODE_PAD_BACK));
   if (e == hipSuccess) e = hipMalloc(&d_units.p, std::max<size_t>(units.size(), 1) * sizeof(ppsk::SketchUnit));
   if (e == hipSuccess) e = hipMalloc(&d_groups.p, std::max<size_t>(groups.size(), 1) * sizeof(ppsk::SketchGroup));
   if (e == hipSuccess) e = hipMalloc(&d_k.p, n_k * sizeof(int32_t));
@@ -105,18 +110,20 @@ int ppsk_sketch(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
   if (e == hipSuccess) e = hipMalloc(&d_raw.p, raw_words * sizeof(uint64_t));
   if (e == hipSuccess) e = hipMalloc(&d_mh.p, (size_t)n_samples * n_k * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&d_dens.p, (size_t)n_samples * n_k * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc(&d_next.p, sizeof(int));
   if (e != hipSuccess) {
     return fail(PPSK_ERR_MEMORY, std::string("sketch buffers: ") + hipGetErrorString(e));
   }
+  uint8_t* codes = (uint8_t*)d_seq.p + ppsk::CODE_PAD_FRONT;
   if (seq_bytes > 0) {
-    HIP_TRY(hipMemcpyAsync(d_seq.p, seq, seq_bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(codes, seq, seq_bytes, hipMemcpyHostToDevice, c->stream));
   }
   HIP_TRY(hipMemcpyAsync(d_units.p, units.data(), units.size() * sizeof(ppsk::SketchUnit),
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(d_groups.p, groups.data(), groups.size() * sizeof(ppsk::SketchGroup),
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(d_k.p, klist, n_k * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  a.seq = (const char*)d_seq.p;
+  a.codes = codes;
   a.units = (const ppsk::SketchUnit*)d_units.p;
   a.groups = (const ppsk::SketchGroup*)d_groups.p;
   a.klist = (const int32_t*)d_k.p;
@@ -128,11 +135,14 @@ int ppsk_sketch(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
   // bin minima in LDS when one sample's bins fit a workgroup's LDS
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device_id));
-  const bool lds_bins = (size_t)nbins * sizeof(uint64_t) <= prop.sharedMemPerBlock;
+  // (the step table takes another 400 bytes)
+  const bool lds_bins = (size_t)nbins * sizeof(uint64_t) + 512 <= prop.sharedMemPerBlock;
+  if (a.bin_recip >> 32) return fail(PPSK_ERR_ARG, "bin reciprocal out of range");
   hipEvent_t e0 = pool_event(c, 0), e1 = pool_event(c, 1);
   if (!e0 || !e1) return fail(PPSK_ERR_HIP, "event creation failed");
   HIP_TRY(hipEventRecord(e0, c->stream));
-  HIP_TRY(ppsk::launch_sketch(a, (int64_t)groups.size(), lds_bins, c->stream));
+  HIP_TRY(ppsk::launch_sketch_decode(codes, seq_bytes, c->stream));
+  HIP_TRY(ppsk::launch_sketch(a, (int64_t)groups.size(), lds_bins, (int*)d_next.p, c->stream));
   HIP_TRY(hipEventRecord(e1, c->stream));
   HIP_TRY(hipMemcpyAsync(raw_out, d_raw.p, raw_words * sizeof(uint64_t), hipMemcpyDeviceToHost,
                          c->stream));

@@ -7,8 +7,14 @@
 
 namespace ppsk {
 
-constexpr int SKETCH_TB = 256;  // threads per hashing workgroup
+constexpr int SKETCH_TB = 512;  // threads per hashing workgroup
 constexpr int BBITS_SK = 14;    // bit layers written per bin
+constexpr int SKETCH_MAX_K = 255;
+// Base codes (one byte per base, 0 A, 1 C, 2 G, 3 T/U, 4 other) are kept
+// with this many bytes of code 4 before and after the sequence, so that the
+// per-lane 16-byte stream loads never leave the buffer.
+constexpr int64_t CODE_PAD_FRONT = SKETCH_MAX_K + 17;
+constexpr int64_t CODE_PAD_BACK = 64;
 
 // A run of n k-mer start positions of one contig: windows start at
 // seq[pos], seq[pos + 1], ..., seq[pos + n - 1] and never cross the contig.
@@ -26,7 +32,7 @@ struct SketchGroup {
 };
 
 struct SketchArgs {
-  const char* seq;           // every sample's contigs, back to back
+  const uint8_t* codes;      // base codes; codes[0] is the first base, padded both sides
   const SketchUnit* units;   // per k-mer length index: its own unit list
   const SketchGroup* groups;
   const int32_t* klist;      // [n_k]
@@ -35,17 +41,24 @@ struct SketchArgs {
   int32_t use_rc;
   int64_t n_samples;
   uint64_t binsize;          // ceil((2^61 - 1) / (64 s64))
-  uint64_t bin_recip;        // floor((2^64 - 1) / binsize)
+  uint64_t bin_recip;        // floor((2^64 - 1) / binsize), < 2^32
   uint64_t* signs;           // [n_samples][n_k][64 s64] bin minima (work)
   uint64_t* raw;             // [n_samples][n_k][s64 * 14] output words
   double* inv_minhash;       // [n_samples][n_k] bin 0 / (2^61 - 1), before densifying
   int32_t* densified;        // [n_samples][n_k] 0, 1, or -1 (every bin empty)
 };
 
+// In place: the bytes codes[0, n) become their base codes; the
+// CODE_PAD_FRONT bytes before and CODE_PAD_BACK bytes after become code 4.
+// codes must be 16-byte aligned.
+hipError_t launch_sketch_decode(uint8_t* codes, int64_t n, hipStream_t stream);
+
 // Clears the bin minima, hashes every group, densifies and bit-slices.
 // lds_bins: keep each workgroup's bin minima in LDS (64 s64 * 8 bytes of
-// dynamic LDS); otherwise update the global minima directly.
-hipError_t launch_sketch(const SketchArgs& a, int64_t n_groups, bool lds_bins,
+// dynamic LDS); otherwise update the global minima directly.  The hashing
+// grid is as many workgroups as fit the device at once; they take groups
+// from the counter next_group (one int of device memory, zeroed here).
+hipError_t launch_sketch(const SketchArgs& a, int64_t n_groups, bool lds_bins, int* next_group,
                          hipStream_t stream);
 
 }  // namespace ppsk
