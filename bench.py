@@ -47,13 +47,13 @@ def parse():
                     help="samples of the CPU baseline's all-vs-all sample")
     ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                     help="run one plain copy kernel of BYTES and exit (HBM counter calibration)")
-    ap.add_argument("--workload", default="plain", choices=["plain", "large", "batch", "serve", "sketch", "genomes"],
+    ap.add_argument("--workload", default="plain", choices=["plain", "large", "batch", "serve", "sketch", "reads", "genomes"],
                     help="plain: 8,192-sample all-vs-all (the headline line); large: "
                          "32,768-sample self pass streamed to host + kNN; batch: 256 uneven "
                          "query-vs-ref jobs; serve: resident 200,000-sample database under "
                          "Poisson query batches; sketch: GPU sketching of 64 synthetic 2 Mb "
-                         "genomes; genomes: the same genomes through constructDatabase + "
-                         "queryDatabase")
+                         "genomes; reads: count-filtered sketching of a 30x read set; "
+                         "genomes: the 64 genomes through constructDatabase + queryDatabase")
     ap.add_argument("--serve-refs", type=int, default=200_000)
     ap.add_argument("--serve-qps", type=float, default=0.0,
                     help="Poisson arrival rate (0: 70%% of the measured saturation rate)")
@@ -136,7 +136,7 @@ def main():
     if a.workload != "plain":
         import bench_workloads as W
         line = {"large": W.large, "batch": W.batch, "serve": W.serve,
-                "sketch": W.sketch, "genomes": W.genomes}[a.workload](a)
+                "sketch": W.sketch, "reads": W.reads, "genomes": W.genomes}[a.workload](a)
         print(json.dumps(line), flush=True)
         return
     import ppsk_hip as H

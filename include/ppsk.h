@@ -321,6 +321,25 @@ int ppsk_sketch(int device_id, int64_t n_samples, const char *seq,
                 uint64_t *raw_out, double *inv_minhash, int32_t *densified);
 
 /*
+ * ppsk_sketch for read sets, with the reference's count filter
+ * (src/sketch/sketch.cpp:128-149, src/sketch/countmin.cpp:37-85): every
+ * valid k-mer occurrence is counted in sequence order and binned only if its
+ * count so far reaches min_count (1..255).  exact = 0: the 4-row count-min
+ * sketch of 2^27 - 1 cells per row, with the reference's second hash (that
+ * of the first k-mer of the current run of valid k-mers) and its flat cell
+ * layout; exact = 1: counts per 64-bit hash (HashCounter).  At most
+ * 2^29 - 1 windows per sample and k.  Replaces the reads branch of sketch()
+ * and the GPU count-min sketcher (src/gpu/sketch.cu:140-176,
+ * src/gpu/gpu_countmin.cu).
+ */
+int ppsk_sketch_filtered(int device_id, int64_t n_samples, const char *seq,
+                         int64_t seq_bytes, const int64_t *contig_off,
+                         int64_t n_contigs, const int64_t *sample_contig,
+                         int32_t n_k, const int32_t *klist, int32_t sketchsize64,
+                         int32_t use_rc, int32_t min_count, int32_t exact,
+                         uint64_t *raw_out, double *inv_minhash, int32_t *densified);
+
+/*
  * Seeded synthetic sketches (no reference counterpart: a generator for tests
  * and benchmarks, writing the reference word layout directly).  Bins evolve
  * down a random binary tree: a child keeps its parent's 14-bit bin value with

@@ -28,6 +28,7 @@
 #include <cstring>
 #include <limits>
 #include <queue>
+#include <unordered_map>
 #include <vector>
 
 #ifdef _OPENMP
@@ -503,13 +504,80 @@ static uint64_t orc_univhash(uint64_t s, uint64_t t) {
  * [off[c], off[c+1]).  usigs: [s64 * 14].  Returns the densify code
  * (0 none, 1 densified, -1 every bin empty); *inv_minhash = bin 0 / (2^61-1)
  * before densification (sketch.cpp:72-75). */
+int orc_sketch_filtered(const char *seq, const long long *off, int n_contigs, int k, int s64,
+                        int use_rc, int min_count, int exact, uint64_t *usigs,
+                        double *inv_minhash);
+
 int orc_sketch(const char *seq, const long long *off, int n_contigs, int k, int s64,
                int use_rc, uint64_t *usigs, double *inv_minhash) {
+  return orc_sketch_filtered(seq, off, n_contigs, k, s64, use_rc, 0, 0, usigs, inv_minhash);
+}
+
+/* ---------------------------------------------------------------------------
+ * Read filtering -- sketch/countmin.cpp:37-85 with countmin.hpp:19-23 and the
+ * multi-hash of sketch/nthash.hpp:180-240.  With min_count > 0 every valid
+ * k-mer occurrence, in sequence order, is counted before it may be binned,
+ * and it is binned only if the count after this occurrence is >= min_count.
+ *  - exact: counts per 64-bit hash value (HashCounter), saturating at 255.
+ *  - count-min: 4 rows of 2^27 - 1 uint8 cells laid out back to back; row r
+ *    takes 27 bits of hash r / 2 (bits 0-26 for even r, 27-53 for odd r),
+ *    where hash 1 = mix(hash 0) = t ^ (t >> 27), t = hash0 * (1 ^ k * seed).
+ *    The hash iterator computes hash 1 only when it (re)starts, at the first
+ *    valid k-mer of a sequence and after each invalid base
+ *    (sketch/stHashIterator.hpp:64-112: init() asks NTMC64 for h2 = 2 hashes,
+ *    next() for h = 1): hash 1 is that of the first k-mer of the current run
+ *    of valid k-mers.
+ *    The cell index can equal 2^27 - 1, one past its row: the flat position
+ *    then is the next row's cell 0 (the reference's std::array of arrays),
+ *    and for the last row one past the table (modelled here as one more
+ *    cell starting at 0).  A cell below 255 is incremented; the estimate is
+ *    the minimum of the incremented values, 255 if every cell is saturated.
+ * ------------------------------------------------------------------------- */
+static const uint64_t ORC_CM_MASK = 0x7FFFFFFULL;
+static const uint64_t ORC_MULTI_SEED = 0x90b45d39fb6da1faULL;
+
+static uint64_t orc_multi_hash(uint64_t h0, unsigned k, unsigned i) {
+  uint64_t t = h0 * ((uint64_t)i ^ (uint64_t)k * ORC_MULTI_SEED);
+  return t ^ (t >> 27);
+}
+
+struct OrcCounter {
+  int exact;
+  unsigned k;
+  std::unordered_map<uint64_t, uint8_t> cells;  // flat cell (count-min) or hash (exact)
+
+  unsigned add(uint64_t h0, uint64_t h1) {
+    if (exact) {
+      uint8_t &c = cells[h0];
+      if (c < 255) ++c;
+      return c;
+    }
+    unsigned est = 255;
+    for (unsigned r = 0; r < 4; ++r) {
+      const uint64_t hv = r < 2 ? h0 : h1;
+      const uint64_t cell = (hv >> (27 * (r % 2))) & ORC_CM_MASK;
+      uint8_t &c = cells[r * ORC_CM_MASK + cell];
+      if (c < 255) {
+        ++c;
+        if (c < est) est = c;
+      }
+    }
+    return est;
+  }
+};
+
+int orc_sketch_filtered(const char *seq, const long long *off, int n_contigs, int k, int s64,
+                        int use_rc, int min_count, int exact, uint64_t *usigs,
+                        double *inv_minhash) {
   const uint64_t nbins = (uint64_t)s64 * 64;
   const uint64_t binsize = (ORC_SIGN_MOD + nbins - 1ULL) / nbins;
   std::vector<uint64_t> signs(nbins, UINT64_MAX);
+  if (min_count > 255) min_count = 255;  // create_sketches truncates to uint8
+  OrcCounter counter{exact, (unsigned)k, {}};
   for (int c = 0; c < n_contigs; ++c) {
     const long long a = off[c], len = off[c + 1] - off[c];
+    bool prev_ok = false;
+    uint64_t run_h1 = 0;
     for (long long p = 0; p + k <= len; ++p) {
       uint64_t fh = 0, rh = 0;
       bool ok = true;
@@ -520,8 +588,15 @@ int orc_sketch(const char *seq, const long long *off, int n_contigs, int k, int 
         fh ^= orc_rot(sd, (unsigned)(k - 1 - i));
         rh ^= orc_rot(orc_seed_comp(ch), (unsigned)i);
       }
-      if (!ok) continue;
-      const uint64_t h = ((use_rc && rh < fh) ? rh : fh) % ORC_SIGN_MOD;
+      if (!ok) {
+        prev_ok = false;
+        continue;
+      }
+      const uint64_t h0 = (use_rc && rh < fh) ? rh : fh;
+      if (!prev_ok) run_h1 = orc_multi_hash(h0, (unsigned)k, 1);
+      prev_ok = true;
+      if (min_count > 0 && (int)counter.add(h0, run_h1) < min_count) continue;
+      const uint64_t h = h0 % ORC_SIGN_MOD;
       const uint64_t bin = h / binsize;
       if (h < signs[bin]) signs[bin] = h;
     }

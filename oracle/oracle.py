@@ -60,6 +60,10 @@ def _load():
     lib.orc_sketch.restype = _c.c_int
     lib.orc_sketch.argtypes = [_c.c_char_p, _p, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _p,
                                _c.POINTER(_c.c_double)]
+    lib.orc_sketch_filtered.restype = _c.c_int
+    lib.orc_sketch_filtered.argtypes = [_c.c_char_p, _p, _c.c_int, _c.c_int, _c.c_int,
+                                        _c.c_int, _c.c_int, _c.c_int, _p,
+                                        _c.POINTER(_c.c_double)]
     lib.orc_knn_rows.restype = None
     lib.orc_knn_rows.argtypes = [_p, _p, _ll, _ll, _c.c_int, _p, _p, _p, _c.c_int]
     lib.orc_square_to_condensed.restype = _ll
@@ -254,6 +258,22 @@ def knn_from_square(square, kNN, num_threads=None):
     return iv, jv, dv
 
 
+def sketch_filtered(contigs, k, sketchsize64, use_rc=True, min_count=0, exact=False):
+    """sketch() with the reads count filter (sketch/countmin.cpp): a k-mer
+    occurrence is binned only if its count so far reaches min_count."""
+    L = lib()
+    bs = [c.encode() if isinstance(c, str) else bytes(c) for c in contigs]
+    seq = b"".join(bs)
+    off = np.zeros(len(bs) + 1, np.int64)
+    off[1:] = np.cumsum([len(b) for b in bs])
+    us = np.zeros(sketchsize64 * 14, np.uint64)
+    mh = _c.c_double()
+    dens = L.orc_sketch_filtered(seq, _ptr(off), len(bs), int(k), int(sketchsize64),
+                                 int(bool(use_rc)), int(min_count), int(bool(exact)), _ptr(us),
+                                 _c.byref(mh))
+    return us, mh.value, dens
+
+
 def sketch(contigs, k, sketchsize64, use_rc=True):
     """sketch/sketch.cpp:118-176 for one sample and one k.  contigs: list of
     str/bytes.  Returns (usigs [s64*14] uint64, inv_minhash, densify code)."""
@@ -289,8 +309,9 @@ def ref_sketch_lib():
         L.ref_read_seqs.restype = _c.c_int64
         L.ref_read_seqs.argtypes = [_c.c_char_p, _c.c_int, _c.c_char_p, _c.c_uint64, _p,
                                     _c.c_uint64]
-        L.ref_sketch_file.argtypes = [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _p,
-                                      _c.POINTER(_c.c_double), _c.POINTER(_c.c_int)]
+        L.ref_sketch_file.argtypes = [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int,
+                                      _c.c_int, _c.c_int, _p, _c.POINTER(_c.c_double),
+                                      _c.POINTER(_c.c_int)]
         _REF_SKETCH = L
     return _REF_SKETCH
 
@@ -335,15 +356,17 @@ def ref_read_seqs(path, kmer_len, cap=1 << 26, max_seqs=1 << 20):
     return out
 
 
-def ref_sketch_file(path, keep_len, k, sketchsize64, use_rc=True):
+def ref_sketch_file(path, keep_len, k, sketchsize64, use_rc=True, min_count=0, exact=False):
     """The reference's sketch of one file at one k (sequences kept if at
-    least keep_len long); None when it keeps no sequence."""
+    least keep_len long; reads filtered by count when min_count > 0); None
+    when it keeps no sequence."""
     L = ref_sketch_lib()
     us = np.zeros(sketchsize64 * 14, np.uint64)
     mh = _c.c_double()
     dn = _c.c_int()
     rc = L.ref_sketch_file(path.encode(), int(keep_len), int(k), int(sketchsize64),
-                           int(bool(use_rc)), _ptr(us), _c.byref(mh), _c.byref(dn))
+                           int(bool(use_rc)), int(min_count), int(bool(exact)), _ptr(us),
+                           _c.byref(mh), _c.byref(dn))
     if rc < 0:
         raise RuntimeError("reference sketch failed")
     if rc == 1:

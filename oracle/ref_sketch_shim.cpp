@@ -82,15 +82,19 @@ extern "C" int64_t ref_read_seqs(const char *path, int kmer_len, char *out, uint
 
 // The reference's Reference sketch of one file at one k-mer length
 // (SeqBuf from the file keeping sequences at least keep_len long, then
-// sketch() with min_count 0, as constructDatabase runs it by default).
-// Returns 1 when the file holds no kept sequence (Reference throws there).
+// sketch(); min_count > 0 filters reads by count, exact picks the hash
+// counter over the count-min sketch).  Returns 1 when the file holds no kept
+// sequence (Reference throws there).
 extern "C" int ref_sketch_file(const char *path, int keep_len, int k, int s64, int use_rc,
-                               uint64_t *usigs, double *inv_minhash, int *densified) {
+                               int min_count, int exact, uint64_t *usigs, double *inv_minhash,
+                               int *densified) {
   try {
     SeqBuf buf(std::vector<std::string>{path}, (size_t)keep_len);
     if (buf.nseqs() == 0) return 1;
     KmerSeeds seeds = generate_seeds({(size_t)k}, false);
-    auto res = sketch(buf, (uint64_t)s64, seeds[(size_t)k], 14, false, use_rc != 0, 0, false);
+    if (min_count > 255) min_count = 255;
+    auto res = sketch(buf, (uint64_t)s64, seeds[(size_t)k], 14, false, use_rc != 0,
+                      (uint8_t)min_count, exact != 0);
     const std::vector<uint64_t> &u = std::get<0>(res);
     std::memcpy(usigs, u.data(), u.size() * sizeof(uint64_t));
     *inv_minhash = std::get<1>(res);

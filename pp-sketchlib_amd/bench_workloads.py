@@ -1,5 +1,5 @@
 """Secondary benchmark workloads of bench.py (--workload large | batch |
-serve | sketch | genomes).
+serve | sketch | reads | genomes).
 
 Each returns one JSON-ready dict.  Inputs of the distance workloads are
 seeded synthetic sketches from the library's tree generator, resident in HBM
@@ -23,6 +23,8 @@ serve   a resident database (default 200,000 samples, 8 k, 10,000 bins,
         (each request alone: one query-vs-ref ppsk_dense into device memory).
 sketch  the same 64 genomes held in memory, sketched by ppsk_sketch (8 k,
         10,000 bins): the GPU sketcher alone, inputs from host memory.
+reads   one read set (150 bp reads at 30x of a 2 Mb genome, 1% errors)
+        sketched with the count filter (min_count 3, count-min sketch).
 genomes 64 synthetic 2 Mb genomes down a random tree, written as FASTA:
         constructDatabase (read, sketch on the GPU at 8 k-mer lengths and
         10,000 bins, write HDF5) and queryDatabase over the 2,016 pairs.
@@ -433,3 +435,52 @@ def sketch(a):
         "config": {"workload": "64 tree genomes of 2 Mb (+0-20% accessory) in host memory",
                    "bases": bases, "kmers": KL8, "bins": S64 * 64, "kmer_windows": kmers},
     }
+
+
+def reads(a):
+    """ppsk_sketch_filtered over one read set: 150 bp reads at 30x of a 2 Mb
+    genome (1% substitution errors), min_count 3, count-min sketch."""
+    rng = np.random.default_rng(23)
+    length, read_len, cover = 2_000_000, 150, 30
+    genome = rng.integers(0, 4, size=length, dtype=np.uint8)
+    n_reads = length * cover // read_len
+    starts = rng.integers(0, length - read_len + 1, size=n_reads)
+    idx = starts[:, None] + np.arange(read_len)[None, :]
+    codes = genome[idx]
+    err = rng.random(codes.shape) < 0.01
+    codes[err] = (codes[err] + rng.integers(1, 4, size=int(err.sum()), dtype=np.uint8)) % 4
+    letters = np.frombuffer(b"ACGT", np.uint8)
+    flat = letters[codes].tobytes()
+    contigs = [flat[i * read_len:(i + 1) * read_len] for i in range(n_reads)]
+    windows = sum(n_reads * (read_len - k + 1) for k in KL8)
+    print(f"# {n_reads} reads, {windows / 1e9:.3f} G windows over 8 k", flush=True)
+    walls, kms = [], []
+    for rep in range(max(2, a.steps)):
+        t = time.perf_counter()
+        H.sketch([contigs], KL8, S64, min_count=3)
+        walls.append(time.perf_counter() - t)
+        kms.append(H.last_timing()[0])
+        print(f"# rep {rep}: wall {walls[-1] * 1e3:.1f} ms, device {kms[-1]:.2f} ms", flush=True)
+    ms, wall = min(kms[1:]), min(walls[1:])
+    line = {
+        "metric": "read k-mer windows/s, GPU sketching with the count filter (min_count 3)",
+        "value": round(windows / (ms * 1e-3), 1), "unit": "windows/s", "n_gpus": 1,
+        "device_ms": round(ms, 2), "wall_ms": round(wall * 1e3, 2),
+        "config": {"workload": "150 bp reads, 30x of a 2 Mb genome, 1% errors, count-min",
+                   "reads": n_reads, "kmers": KL8, "bins": S64 * 64, "windows": windows},
+    }
+    if not a.no_cpu:
+        import sys
+        sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(
+            os.path.abspath(__file__))), "oracle"))
+        import oracle as O
+        sub = contigs[: n_reads // 20]
+        t = time.perf_counter()
+        O.sketch_filtered(sub, 15, S64, True, 3, False)
+        dt = time.perf_counter() - t
+        w_sub = len(sub) * (read_len - 15 + 1)
+        line["cpu_baseline"] = {"value": round(w_sub / dt, 1), "unit": "windows/s", "cores": 1,
+                                "kind": "port",
+                                "sample": f"the oracle's filtered sketch of 1/20 of the reads at "
+                                          f"k = 15 ({dt:.1f} s)"}
+    return line

@@ -130,8 +130,8 @@ SequenceStats read_sequence_stats(const std::vector<std::string>& files, size_t 
 void construct_database(const std::string& db_name, const std::vector<std::string>& names,
                         const std::vector<std::vector<std::string>>& files,
                         std::vector<size_t> kmer_lengths, size_t sketchsize64, bool codon_phased,
-                        bool calc_random, bool use_rc, size_t min_count, size_t num_threads,
-                        int device_id);
+                        bool calc_random, bool use_rc, size_t min_count, bool exact,
+                        size_t num_threads, int device_id);
 
 void write_synthetic_db(const std::string& db_name, int64_t n_samples,
                         const std::vector<size_t>& kmer_lengths, size_t sketchsize64,

@@ -155,16 +155,15 @@ py::tuple sparsifyDists(const farray&, float, int, size_t) {
       "Downgrade to v2.0.0, or update the calling package");
 }
 
-// Sketches on the GPU whatever use_gpu says (there is no CPU path); exact
-// read counting is not offered (exact only matters with min_count > 0).
+// Sketches on the GPU whatever use_gpu says (there is no CPU path).
 void constructDatabase(const std::string& db_name, const std::vector<std::string>& names,
                        const std::vector<std::vector<std::string>>& files,
                        std::vector<size_t> kmer_lengths, size_t sketch_size, bool codon_phased,
-                       bool calc_random, bool use_rc, size_t min_count, bool /*exact*/,
+                       bool calc_random, bool use_rc, size_t min_count, bool exact,
                        size_t num_threads, bool /*use_gpu*/, int device_id) {
   py::gil_scoped_release release;
   ppsk_host::construct_database(db_name, names, files, kmer_lengths, sketch_size, codon_phased,
-                                calc_random, use_rc, min_count, num_threads, device_id);
+                                calc_random, use_rc, min_count, exact, num_threads, device_id);
 }
 
 // addRandomToDb (src/sketchlib_bindings.cpp:189-202): Monte Carlo tables from

@@ -159,14 +159,14 @@ bool file_exists(const std::string& p) {
 std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::string>& names,
                                  const std::vector<std::vector<std::string>>& files,
                                  const std::vector<size_t>& kmer_lengths, size_t sketchsize64,
-                                 bool use_rc, size_t min_count, size_t num_threads,
+                                 bool use_rc, size_t min_count, bool exact, size_t num_threads,
                                  int device_id);
 
 void construct_database(const std::string& db_name, const std::vector<std::string>& names,
                         const std::vector<std::vector<std::string>>& files,
                         std::vector<size_t> kmer_lengths, size_t sketchsize64, bool codon_phased,
-                        bool calc_random, bool use_rc, size_t min_count, size_t num_threads,
-                        int device_id) {
+                        bool calc_random, bool use_rc, size_t min_count, bool exact,
+                        size_t num_threads, int device_id) {
   if (codon_phased) {
     throw std::runtime_error("Codon phased seeds not yet implemented for GPU sketching");
   }
@@ -184,7 +184,7 @@ void construct_database(const std::string& db_name, const std::vector<std::strin
     resketch = all.size() != names.size();
   }
   if (resketch) {
-    all = sketch_files(h5, names, files, kmer_lengths, sketchsize64, use_rc, min_count,
+    all = sketch_files(h5, names, files, kmer_lengths, sketchsize64, use_rc, min_count, exact,
                        num_threads, device_id);
   }
   // constructDatabase (src/sketchlib_bindings.cpp:77-86)
@@ -201,7 +201,7 @@ void construct_database(const std::string& db_name, const std::vector<std::strin
 std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::string>& names,
                                  const std::vector<std::vector<std::string>>& files,
                                  const std::vector<size_t>& kmer_lengths, size_t sketchsize64,
-                                 bool use_rc, size_t min_count, size_t num_threads,
+                                 bool use_rc, size_t min_count, bool exact, size_t num_threads,
                                  int device_id) {
   std::cerr << "Sketching " << names.size() << " genomes on GPU device " << device_id << std::endl;
   std::vector<SampleSeq> seqs(names.size());
@@ -211,9 +211,6 @@ std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::s
     try {
       for (const auto& f : files[i]) read_file(f, kmer_lengths.back(), seqs[i]);
       if (seqs[i].contigs.empty()) errors[i] = names[i] + " contains no sequence";
-      if (seqs[i].reads && min_count > 0) {
-        errors[i] = "read filtering (min_count > 0) is not supported by this build: " + names[i];
-      }
     } catch (const std::exception& e) {
       errors[i] = e.what();
     }
@@ -233,12 +230,18 @@ std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::s
   std::vector<int32_t> kl(kmer_lengths.begin(), kmer_lengths.end());
   const size_t K = kl.size(), words = sketchsize64 * 14;
   write_new_db(h5, use_rc, false, SKETCH_VERSION);
-  // batches of samples bounded by their bases (device copy of the sequence)
+  // reads are counted before binning when min_count > 0 (sketch.cpp:128-149;
+  // create_sketches truncates min_count to uint8, api.cpp:66-69)
+  const int32_t mc = (int32_t)std::min<size_t>(min_count, 255);
+  auto filtered = [&](size_t s) { return seqs[s].reads && mc > 0; };
+  // batches of samples of one kind (count-filtered or not), bounded by their
+  // bases (device copy of the sequence)
   const size_t batch_bases = (size_t)1 << 31;
   size_t s0 = 0;
   while (s0 < seqs.size()) {
     size_t s1 = s0, bases = 0;
-    while (s1 < seqs.size() && (s1 == s0 || bases + seqs[s1].total <= batch_bases)) {
+    while (s1 < seqs.size() && (s1 == s0 || (bases + seqs[s1].total <= batch_bases &&
+                                             filtered(s1) == filtered(s0)))) {
       bases += seqs[s1].total;
       ++s1;
     }
@@ -256,10 +259,16 @@ std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::s
     std::vector<uint64_t> raw((size_t)n * K * words);
     std::vector<double> mh(n);
     std::vector<int32_t> dn(n);
-    const int rc = ppsk_sketch(device_id, n, seq.data(), (int64_t)seq.size(), off.data(),
-                               (int64_t)off.size() - 1, sc.data(), (int32_t)K, kl.data(),
-                               (int32_t)sketchsize64, use_rc ? 1 : 0, raw.data(), mh.data(),
-                               dn.data());
+    const int rc =
+        filtered(s0)
+            ? ppsk_sketch_filtered(device_id, n, seq.data(), (int64_t)seq.size(), off.data(),
+                                   (int64_t)off.size() - 1, sc.data(), (int32_t)K, kl.data(),
+                                   (int32_t)sketchsize64, use_rc ? 1 : 0, mc, exact ? 1 : 0,
+                                   raw.data(), mh.data(), dn.data())
+            : ppsk_sketch(device_id, n, seq.data(), (int64_t)seq.size(), off.data(),
+                          (int64_t)off.size() - 1, sc.data(), (int32_t)K, kl.data(),
+                          (int32_t)sketchsize64, use_rc ? 1 : 0, raw.data(), mh.data(),
+                          dn.data());
     if (rc != PPSK_OK) throw std::runtime_error(ppsk_last_error());
     std::vector<Sketch> out;
     for (int64_t x = 0; x < n; ++x) {

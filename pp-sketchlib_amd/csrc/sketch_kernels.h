@@ -61,4 +61,34 @@ hipError_t launch_sketch_decode(uint8_t* codes, int64_t n, hipStream_t stream);
 hipError_t launch_sketch(const SketchArgs& a, int64_t n_groups, bool lds_bins, int* next_group,
                          hipStream_t stream);
 
+// Bin-minima reset and the densify / bit-slice pass of launch_sketch, for
+// callers that fill the minima themselves (run_reads_job).
+hipError_t launch_sketch_reset(const SketchArgs& a, hipStream_t stream);
+hipError_t launch_sketch_finish(const SketchArgs& a, hipStream_t stream);
+
+// One (sample, k) of read sketching with a count filter: units cover the
+// sample's contigs for this k (pad = 1 for the first unit of a contig),
+// wbase[u] is the index of unit u's first window among the n_windows
+// windows of the sample, in sequence order.
+constexpr int64_t READS_MAX_WINDOWS = ((int64_t)1 << 29) - 1;  // 4 events each fit an int count
+struct ReadsJob {
+  const uint8_t* codes;
+  const SketchUnit* units;
+  const int64_t* wbase;
+  int64_t n_units;
+  int64_t n_windows;
+  int k;
+  int use_rc;
+  int min_count;  // 1 .. 255
+  int exact;      // hash counter instead of the count-min sketch
+  uint64_t binsize;
+  uint32_t recip;
+  uint64_t* bins;  // this (sample, k)'s bin minima
+};
+// Device workspace a job of up to max_windows windows needs.
+size_t reads_workspace_bytes(int64_t max_windows);
+// Hashes, counts and bins one job into j.bins (synchronises the stream once
+// to learn the occurrence count).
+hipError_t run_reads_job(const ReadsJob& j, void* ws, size_t ws_bytes, hipStream_t stream);
+
 }  // namespace ppsk

@@ -30,6 +30,8 @@
 #include <algorithm>
 #include <cstdint>
 
+#include <hipcub/hipcub.hpp>
+
 #include "sketch_kernels.h"
 
 namespace ppsk {
@@ -126,6 +128,96 @@ __global__ void sketch_decode_kernel(uint8_t* codes, int64_t n) {
   for (int64_t x = t0; x < CODE_PAD_BACK; x += stride) codes[n + x] = 4;
 }
 
+// The 25 step terms for k: entry cin * 5 + cout (codes 4 contribute 0).
+__device__ __forceinline__ StepTerms step_terms(int e, int k) {
+  const int cin = e / 5, cout = e % 5;
+  StepTerms t;
+  t.f = (cin < 4 ? seed_of(cin) : 0ULL) ^ (cout < 4 ? rotj(seed_of(cout), k) : 0ULL);
+  t.r = (cout < 4 ? seed_of(3 - cout) : 0ULL) ^ (cin < 4 ? rotj(seed_of(3 - cin), k) : 0ULL);
+  return t;
+}
+
+// One lane rolls through one unit: emit(p, h) for every window p (0-based in
+// the unit, ascending) free of code-4 bases, h its ntHash (the smaller of
+// the two strands with use_rc).  Step i rolls in base i of the unit and,
+// from i = k on, rolls out base i - k; step i ends the window starting at
+// i - k + 1.  Bases come as two streams of 16-byte loads issued a block
+// ahead; each half block's step terms are read from LDS before its emits.
+template <class Emit>
+__device__ __forceinline__ void roll_unit(const uint8_t* codes, const SketchUnit su, int k,
+                                          bool use_rc, const StepTerms* terms, Emit&& emit) {
+  const int steps = su.n + k - 1;
+  const uintptr_t ia = (uintptr_t)(codes + su.pos), oa = ia - (uintptr_t)k;
+  const uint32_t* ip = reinterpret_cast<const uint32_t*>(ia & ~(uintptr_t)3);
+  const uint32_t* op = reinterpret_cast<const uint32_t*>(oa & ~(uintptr_t)3);
+  const uint32_t ish = (uint32_t)(ia & 3), osh = (uint32_t)(oa & 3);
+  uint32_t ic[4], in_[4], oc[4], on_[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    ic[q] = ip[q];
+    in_[q] = ip[4 + q];
+    oc[q] = op[q];
+    on_[q] = op[4 + q];
+  }
+  ip += 8;
+  op += 8;
+  uint64_t fh = 0, rh = 0;
+  int last_bad = -1;
+  for (int i0 = 0; i0 < steps; i0 += 16) {
+    uint32_t inn[4], onn[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      inn[q] = ip[q];
+      onn[q] = op[q];
+    }
+    ip += 4;
+    op += 4;
+    uint32_t wi[4], wo[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      wi[q] = __builtin_amdgcn_alignbyte(q < 3 ? ic[q + 1] : in_[0], ic[q], ish);
+      wo[q] = __builtin_amdgcn_alignbyte(q < 3 ? oc[q + 1] : on_[0], oc[q], osh);
+    }
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      StepTerms t[8];
+      uint32_t bad = 0;
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int j = half * 8 + jj;
+        const uint32_t cin = (wi[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+        uint32_t cout = (wo[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+        cout = i0 + j >= k ? cout : 4u;
+        t[jj] = terms[cin * 5 + cout];
+        bad |= (cin == 4u ? 1u : 0u) << jj;
+      }
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int i = i0 + half * 8 + jj;
+        fh = rot1(fh) ^ t[jj].f;
+        rh = rot1_inv(rh ^ t[jj].r);
+        last_bad = (bad >> jj) & 1u ? i : last_bad;
+        if (i - k >= last_bad && i < steps) emit(i - k + 1, (use_rc && rh < fh) ? rh : fh);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ic[q] = in_[q];
+      in_[q] = inn[q];
+      oc[q] = on_[q];
+      on_[q] = onn[q];
+    }
+  }
+}
+
+// bin of h (< 2^61 - 1): quotient from the reciprocal (< 2^32), one correction
+__device__ __forceinline__ uint64_t bin_of(uint64_t h, uint64_t binsize, uint32_t recip) {
+  const uint64_t lo = (uint64_t)(uint32_t)h * recip;
+  uint64_t bin = ((h >> 32) * recip + (lo >> 32)) >> 32;
+  if (h - bin * binsize >= binsize) ++bin;
+  return bin;
+}
+
 template <bool LDS_BINS>
 __global__ __launch_bounds__(SKETCH_TB) void sketch_hash_kernel(SketchArgs a, int64_t n_groups,
                                                                 int* next_group) {
@@ -134,7 +226,6 @@ __global__ __launch_bounds__(SKETCH_TB) void sketch_hash_kernel(SketchArgs a, in
   __shared__ int64_t cur_group;
   __shared__ int cur_k;
   const int64_t nbins = (int64_t)a.s64 * 64;
-  // exact division by binsize: quotient from the reciprocal (< 2^32), one correction
   const uint64_t binsize = a.binsize;
   const uint32_t recip = (uint32_t)a.bin_recip;
   const bool use_rc = a.use_rc != 0;
@@ -151,97 +242,22 @@ __global__ __launch_bounds__(SKETCH_TB) void sketch_hash_kernel(SketchArgs a, in
     const int kk = g.kk;
     const int k = a.klist[kk];
     uint64_t* gbins = a.signs + ((int64_t)g.sample * a.n_k + kk) * nbins;
-    if (k != cur_k) {
-      if (threadIdx.x < 25) {
-        const int cin = threadIdx.x / 5, cout = threadIdx.x % 5;
-        StepTerms t;
-        t.f = (cin < 4 ? seed_of(cin) : 0ULL) ^ (cout < 4 ? rotj(seed_of(cout), k) : 0ULL);
-        t.r = (cout < 4 ? seed_of(3 - cout) : 0ULL) ^ (cin < 4 ? rotj(seed_of(3 - cin), k) : 0ULL);
-        terms[threadIdx.x] = t;
-      }
-    }
+    if (k != cur_k && threadIdx.x < 25) terms[threadIdx.x] = step_terms(threadIdx.x, k);
     if (LDS_BINS) {
       for (int64_t b = threadIdx.x; b < nbins; b += SKETCH_TB) lbins[b] = ~0ULL;
     }
     __syncthreads();
     if (threadIdx.x == 0) cur_k = k;
     for (int64_t u = g.unit0 + threadIdx.x; u < g.unit1; u += SKETCH_TB) {
-      const SketchUnit su = a.units[u];
-      // step i rolls in base i of the unit and, from i = k on, rolls out base
-      // i - k; step i ends the window starting at i - k + 1
-      const int steps = su.n + k - 1;
-      const uintptr_t ia = (uintptr_t)(a.codes + su.pos), oa = ia - (uintptr_t)k;
-      const uint32_t* ip = reinterpret_cast<const uint32_t*>(ia & ~(uintptr_t)3);
-      const uint32_t* op = reinterpret_cast<const uint32_t*>(oa & ~(uintptr_t)3);
-      const uint32_t ish = (uint32_t)(ia & 3), osh = (uint32_t)(oa & 3);
-      uint32_t ic[4], in_[4], oc[4], on_[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        ic[q] = ip[q];
-        in_[q] = ip[4 + q];
-        oc[q] = op[q];
-        on_[q] = op[4 + q];
-      }
-      ip += 8;
-      op += 8;
-      uint64_t fh = 0, rh = 0;
-      int last_bad = -1;
-      for (int i0 = 0; i0 < steps; i0 += 16) {
-        uint32_t inn[4], onn[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          inn[q] = ip[q];
-          onn[q] = op[q];
+      roll_unit(a.codes, a.units[u], k, use_rc, terms, [&](int, uint64_t h0) {
+        const uint64_t h = mod_sign(h0);
+        const uint64_t bin = bin_of(h, binsize, recip);
+        if (LDS_BINS) {
+          atomicMin((unsigned long long*)&lbins[bin], (unsigned long long)h);
+        } else if (h < gbins[bin]) {
+          atomicMin((unsigned long long*)&gbins[bin], (unsigned long long)h);
         }
-        ip += 4;
-        op += 4;
-        uint32_t wi[4], wo[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          wi[q] = __builtin_amdgcn_alignbyte(q < 3 ? ic[q + 1] : in_[0], ic[q], ish);
-          wo[q] = __builtin_amdgcn_alignbyte(q < 3 ? oc[q + 1] : on_[0], oc[q], osh);
-        }
-        // each half block's step terms are read before any of its bin updates
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          StepTerms t[8];
-          uint32_t bad = 0;
-#pragma unroll
-          for (int jj = 0; jj < 8; ++jj) {
-            const int j = half * 8 + jj;
-            const uint32_t cin = (wi[j >> 2] >> (8 * (j & 3))) & 0xFFu;
-            uint32_t cout = (wo[j >> 2] >> (8 * (j & 3))) & 0xFFu;
-            cout = i0 + j >= k ? cout : 4u;
-            t[jj] = terms[cin * 5 + cout];
-            bad |= (cin == 4u ? 1u : 0u) << jj;
-          }
-#pragma unroll
-          for (int jj = 0; jj < 8; ++jj) {
-            const int i = i0 + half * 8 + jj;
-            fh = rot1(fh) ^ t[jj].f;
-            rh = rot1_inv(rh ^ t[jj].r);
-            last_bad = (bad >> jj) & 1u ? i : last_bad;
-            if (i - k >= last_bad && i < steps) {
-              const uint64_t h = mod_sign((use_rc && rh < fh) ? rh : fh);
-              const uint64_t lo = (uint64_t)(uint32_t)h * recip;
-              uint64_t bin = ((h >> 32) * recip + (lo >> 32)) >> 32;
-              if (h - bin * binsize >= binsize) ++bin;
-              if (LDS_BINS) {
-                atomicMin((unsigned long long*)&lbins[bin], (unsigned long long)h);
-              } else if (h < gbins[bin]) {
-                atomicMin((unsigned long long*)&gbins[bin], (unsigned long long)h);
-              }
-            }
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          ic[q] = in_[q];
-          in_[q] = inn[q];
-          oc[q] = on_[q];
-          on_[q] = onn[q];
-        }
-      }
+      });
     }
     if (LDS_BINS) {
       __syncthreads();
@@ -251,6 +267,254 @@ __global__ __launch_bounds__(SKETCH_TB) void sketch_hash_kernel(SketchArgs a, in
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Reads count filter (sketch/countmin.cpp:37-85, sketch/sketch.cpp:128-149):
+// every valid k-mer occurrence is counted, in sequence order, before it may
+// be binned, and is binned only if its count after that occurrence reaches
+// min_count.  Order matters only through the counts, so one (sample, k) job
+// runs as: hash every window (h0, validity) -> occurrence order by prefix
+// sum -> per-occurrence keys -> stable radix sort by key -> an occurrence's
+// count is its rank among equal keys + 1, so "count >= m" is "the key m - 1
+// places back in sorted order is the same" -> bin the passing occurrences.
+//  - exact (HashCounter): key = h0.
+//  - count-min: 4 keys per occurrence, the flat cells of the reference's
+//    4 x (2^27 - 1) table (row r, 27 bits of hash r / 2; an index of
+//    2^27 - 1 lands on the next row's cell 0, as in the reference's nested
+//    std::array); the estimate is the minimum over the rows, so an
+//    occurrence passes when all four of its events pass.  Hash 1 is the mix
+//    of hash 0 of the first k-mer of the current run of valid k-mers: the
+//    reference's hash iterator computes extra hashes only when it restarts
+//    (sketch/stHashIterator.hpp:64-112).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint64_t CM_MASK = 0x7FFFFFFULL;
+constexpr uint64_t MULTI_SEED = 0x90b45d39fb6da1faULL;
+
+__device__ __forceinline__ uint64_t multi_hash1(uint64_t h0, int k) {
+  const uint64_t t = h0 * (1ULL ^ (uint64_t)k * MULTI_SEED);
+  return t ^ (t >> 27);
+}
+
+__global__ __launch_bounds__(256) void reads_hash_kernel(ReadsJob j, uint64_t* h0, uint8_t* st) {
+  __shared__ StepTerms terms[25];
+  if (threadIdx.x < 25) terms[threadIdx.x] = step_terms(threadIdx.x, j.k);
+  __syncthreads();
+  const bool use_rc = j.use_rc != 0;
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < j.n_units;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    const SketchUnit su = j.units[u];
+    const int64_t w0 = j.wbase[u];
+    roll_unit(j.codes, su, j.k, use_rc, terms, [&](int p, uint64_t h) {
+      h0[w0 + p] = h;
+      st[w0 + p] = (p == 0 && su.pad) ? 3 : 1;
+    });
+  }
+}
+
+// rs[w] = w if a run of valid windows starts at w, else -1
+__global__ void reads_run_kernel(const uint8_t* st, int64_t n, int64_t* rs) {
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < n;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    const uint8_t s = st[w];
+    rs[w] = (s != 0 && (s == 3 || w == 0 || st[w - 1] == 0)) ? w : -1;
+  }
+}
+
+// occurrence t = tix[w] of every valid window: H0[t], H1[t]
+__global__ void reads_compact_kernel(const uint8_t* st, const uint64_t* h0, const int64_t* run,
+                                     const int64_t* tix, int64_t n, int k, uint64_t* H0,
+                                     uint64_t* H1) {
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < n;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    if (st[w] != 0) {
+      const int64_t t = tix[w];
+      H0[t] = h0[w];
+      H1[t] = multi_hash1(h0[run[w]], k);
+    }
+  }
+}
+
+__global__ void reads_cm_keys_kernel(const uint64_t* H0, const uint64_t* H1, int64_t n,
+                                     uint32_t* key, uint32_t* val) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < 4 * n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = e >> 2;
+    const uint32_t r = (uint32_t)(e & 3);
+    const uint64_t hv = r < 2 ? H0[t] : H1[t];
+    key[e] = (uint32_t)(r * CM_MASK + ((hv >> (27 * (r & 1))) & CM_MASK));
+    val[e] = (uint32_t)e;
+  }
+}
+
+__global__ void reads_iota_kernel(uint32_t* val, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    val[e] = (uint32_t)e;
+  }
+}
+
+// after the stable sort: the count of element s's key up to and including
+// it is (rank among equal keys) + 1; it reaches m iff the key m - 1 places
+// back is the same
+template <class Key>
+__global__ void reads_pass_kernel(const Key* key, const uint32_t* val, int64_t n, int m,
+                                  uint8_t* pass) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    pass[val[s]] = (s >= m - 1 && key[s - (m - 1)] == key[s]) ? 1 : 0;
+  }
+}
+
+__global__ void reads_bin_kernel(const uint64_t* H0, const uint8_t* pass, int64_t n, int events,
+                                 uint64_t binsize, uint32_t recip, uint64_t* bins) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    bool ok = true;
+    for (int r = 0; r < events; ++r) ok = ok && pass[t * events + r];
+    if (ok) {
+      const uint64_t h = mod_sign(H0[t]);
+      const uint64_t bin = bin_of(h, binsize, recip);
+      if (h < bins[bin]) atomicMin((unsigned long long*)&bins[bin], (unsigned long long)h);
+    }
+  }
+}
+
+struct Max64 {
+  __device__ __forceinline__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; }
+};
+
+struct ValidOne {
+  __device__ __forceinline__ int64_t operator()(uint8_t s) const { return s != 0 ? 1 : 0; }
+};
+
+inline unsigned grid_for(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
+}
+
+// Workspace carving: every region 256-byte aligned.
+struct Carve {
+  char* p;
+  size_t used = 0;
+  template <class T>
+  T* take(int64_t count) {
+    T* r = reinterpret_cast<T*>(p ? p + used : nullptr);
+    used += ((size_t)std::max<int64_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+    return r;
+  }
+};
+
+size_t cub_temp_bytes(int64_t max_windows) {
+  size_t a = 0, b = 0, c = 0, d = 0;
+  const int64_t W = std::max<int64_t>(max_windows, 1);
+  hipcub::TransformInputIterator<int64_t, ValidOne, const uint8_t*> it(nullptr, ValidOne());
+  (void)hipcub::DeviceScan::InclusiveScan(nullptr, a, (const int64_t*)nullptr, (int64_t*)nullptr,
+                                    Max64(), (int)W);
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, it, (int64_t*)nullptr, (int)W);
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, c, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)(4 * W),
+                                     0, 29);
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, d, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)W, 0, 64);
+  return std::max(std::max(a, b), std::max(c, d));
+}
+
+}  // namespace
+
+size_t reads_workspace_bytes(int64_t max_windows) {
+  Carve c{nullptr};
+  const int64_t W = std::max<int64_t>(max_windows, 1);
+  c.take<uint64_t>(W);      // h0
+  c.take<uint8_t>(W);       // st
+  c.take<int64_t>(W);       // rs
+  c.take<int64_t>(W);       // run
+  c.take<int64_t>(W);       // tix
+  c.take<uint64_t>(W);      // H0
+  c.take<uint64_t>(W);      // H1
+  c.take<uint32_t>(4 * W);  // keys in (count-min) / 2 W for exact
+  c.take<uint32_t>(4 * W);  // keys out
+  c.take<uint32_t>(4 * W);  // vals in
+  c.take<uint32_t>(4 * W);  // vals out
+  c.take<uint8_t>(4 * W);   // pass
+  c.take<char>((int64_t)cub_temp_bytes(W));
+  return c.used;
+}
+
+hipError_t run_reads_job(const ReadsJob& j, void* ws, size_t ws_bytes, hipStream_t stream) {
+  const int64_t W = j.n_windows;
+  if (W <= 0) return hipSuccess;
+  if (W > READS_MAX_WINDOWS || reads_workspace_bytes(W) > ws_bytes) return hipErrorInvalidValue;
+  Carve c{static_cast<char*>(ws)};
+  uint64_t* h0 = c.take<uint64_t>(W);
+  uint8_t* st = c.take<uint8_t>(W);
+  int64_t* rs = c.take<int64_t>(W);
+  int64_t* run = c.take<int64_t>(W);
+  int64_t* tix = c.take<int64_t>(W);
+  uint64_t* H0 = c.take<uint64_t>(W);
+  uint64_t* H1 = c.take<uint64_t>(W);
+  uint32_t* kin = c.take<uint32_t>(4 * W);
+  uint32_t* kout = c.take<uint32_t>(4 * W);
+  uint32_t* vin = c.take<uint32_t>(4 * W);
+  uint32_t* vout = c.take<uint32_t>(4 * W);
+  uint8_t* pass = c.take<uint8_t>(4 * W);
+  const size_t temp_bytes = cub_temp_bytes(W);
+  void* temp = c.take<char>((int64_t)temp_bytes);
+
+  hipError_t e = hipMemsetAsync(st, 0, (size_t)W, stream);
+  if (e != hipSuccess) return e;
+  reads_hash_kernel<<<grid_for(j.n_units), 256, 0, stream>>>(j, h0, st);
+  reads_run_kernel<<<grid_for(W), 256, 0, stream>>>(st, W, rs);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  size_t tb = temp_bytes;
+  e = hipcub::DeviceScan::InclusiveScan(temp, tb, rs, run, Max64(), (int)W, stream);
+  if (e != hipSuccess) return e;
+  hipcub::TransformInputIterator<int64_t, ValidOne, const uint8_t*> valid(st, ValidOne());
+  tb = temp_bytes;
+  e = hipcub::DeviceScan::ExclusiveSum(temp, tb, valid, tix, (int)W, stream);
+  if (e != hipSuccess) return e;
+  reads_compact_kernel<<<grid_for(W), 256, 0, stream>>>(st, h0, run, tix, W, j.k, H0, H1);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // occurrence count
+  int64_t last_t = 0;
+  uint8_t last_st = 0;
+  if ((e = hipMemcpyAsync(&last_t, tix + (W - 1), sizeof(int64_t), hipMemcpyDeviceToHost,
+                          stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(&last_st, st + (W - 1), 1, hipMemcpyDeviceToHost, stream)) !=
+          hipSuccess ||
+      (e = hipStreamSynchronize(stream)) != hipSuccess) {
+    return e;
+  }
+  const int64_t N = last_t + (last_st != 0 ? 1 : 0);
+  if (N == 0) return hipSuccess;
+  int events;
+  if (j.exact) {
+    events = 1;
+    uint64_t* k64in = reinterpret_cast<uint64_t*>(kin);
+    uint64_t* k64out = reinterpret_cast<uint64_t*>(kout);
+    e = hipMemcpyAsync(k64in, H0, (size_t)N * sizeof(uint64_t), hipMemcpyDeviceToDevice, stream);
+    if (e != hipSuccess) return e;
+    reads_iota_kernel<<<grid_for(N), 256, 0, stream>>>(vin, N);
+    tb = temp_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(temp, tb, k64in, k64out, vin, vout, (int)N, 0, 64,
+                                           stream);
+    if (e != hipSuccess) return e;
+    reads_pass_kernel<uint64_t><<<grid_for(N), 256, 0, stream>>>(k64out, vout, N, j.min_count,
+                                                                  pass);
+  } else {
+    events = 4;
+    reads_cm_keys_kernel<<<grid_for(4 * N), 256, 0, stream>>>(H0, H1, N, kin, vin);
+    tb = temp_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(temp, tb, kin, kout, vin, vout, (int)(4 * N), 0, 29,
+                                           stream);
+    if (e != hipSuccess) return e;
+    reads_pass_kernel<uint32_t><<<grid_for(4 * N), 256, 0, stream>>>(kout, vout, 4 * N,
+                                                                      j.min_count, pass);
+  }
+  reads_bin_kernel<<<grid_for(N), 256, 0, stream>>>(H0, pass, N, events, j.binsize, j.recip,
+                                                    j.bins);
+  return hipGetLastError();
 }
 
 // Reference densifybin (sketch/sketch.cpp:93-115) and fillusigs (80-91) for
@@ -343,6 +607,18 @@ hipError_t launch_sketch(const SketchArgs& a, int64_t n_groups, bool lds_bins, i
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
+  sketch_finish_kernel<<<(unsigned)(a.n_samples * a.n_k), 64, 0, stream>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_sketch_reset(const SketchArgs& a, hipStream_t stream) {
+  const int64_t total = (int64_t)a.n_samples * a.n_k * a.s64 * 64;
+  fill_u64<<<(unsigned)((total + 255) / 256 > 65536 ? 65536 : (total + 255) / 256), 256, 0,
+             stream>>>(a.signs, total, ~0ULL);
+  return hipGetLastError();
+}
+
+hipError_t launch_sketch_finish(const SketchArgs& a, hipStream_t stream) {
   sketch_finish_kernel<<<(unsigned)(a.n_samples * a.n_k), 64, 0, stream>>>(a);
   return hipGetLastError();
 }

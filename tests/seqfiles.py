@@ -61,3 +61,22 @@ def samples(tmp_path, seed=4):
     files.append([put(tmp_path / "s5.fa", fasta([("c1", g[5])], wrap=80))])
     names = [f"sample_{i}" for i in range(len(files))]
     return names, files
+
+
+def fastq_reads(rng, genome, n_reads, read_len=100, error=0.01, n_rate=0.0):
+    """Reads sampled uniformly from genome (str), either strand, with
+    substitution errors and optional N calls; FASTQ text."""
+    comp = str.maketrans("ACGT", "TGCA")
+    out = []
+    for r in range(n_reads):
+        p = int(rng.integers(0, len(genome) - read_len + 1))
+        s = genome[p:p + read_len]
+        if rng.random() < 0.5:
+            s = s.translate(comp)[::-1]
+        s = np.array(list(s))
+        m = rng.random(read_len) < error
+        s[m] = rng.choice(list("ACGT"), size=int(m.sum()))
+        if n_rate:
+            s[rng.random(read_len) < n_rate] = "N"
+        out.append(f"@r{r}\n{''.join(s)}\n+\n{'I' * read_len}\n")
+    return "".join(out)

@@ -106,7 +106,5 @@ def test_construct_database_errors(pp, tmp_path):
     with pytest.raises(RuntimeError, match="Errors during sketching"):
         pp.constructDatabase(str(tmp_path / "e"), ["x"], [[empty]], KL, S64)
     fq = _put(tmp_path / "r.fq", "@r\n" + "ACGT" * 20 + "\n+\n" + "I" * 80 + "\n")
-    with pytest.raises(RuntimeError, match="Errors during sketching"):
-        pp.constructDatabase(str(tmp_path / "m"), ["r"], [[fq]], KL, S64, min_count=2)
     with pytest.raises(RuntimeError, match="Codon phased"):
         pp.constructDatabase(str(tmp_path / "c"), ["r"], [[fq]], KL, S64, codon_phased=True)
