@@ -321,23 +321,29 @@ int ppsk_sketch(int device_id, int64_t n_samples, const char *seq,
                 uint64_t *raw_out, double *inv_minhash, int32_t *densified);
 
 /*
- * ppsk_sketch for read sets, with the reference's count filter
+ * ppsk_sketch with the reference's two sketching options.
+ * codon_phased: the codon-phased seeds of generate_seeds
+ * (src/sketch/sketch.cpp:43-54): a span of 3k - 2 bases of which every
+ * third (offsets 0, 3, ..., 3(k-1)) is hashed, the span free of non-ACGTU
+ * characters (src/sketch/nthash.hpp:634-722); 3k <= 255.
+ * min_count > 0 (1..255): the reads count filter
  * (src/sketch/sketch.cpp:128-149, src/sketch/countmin.cpp:37-85): every
  * valid k-mer occurrence is counted in sequence order and binned only if its
- * count so far reaches min_count (1..255).  exact = 0: the 4-row count-min
- * sketch of 2^27 - 1 cells per row, with the reference's second hash (that
- * of the first k-mer of the current run of valid k-mers) and its flat cell
- * layout; exact = 1: counts per 64-bit hash (HashCounter).  At most
- * 2^29 - 1 windows per sample and k.  Replaces the reads branch of sketch()
- * and the GPU count-min sketcher (src/gpu/sketch.cu:140-176,
- * src/gpu/gpu_countmin.cu).
+ * count so far reaches min_count.  exact = 0: the 4-row count-min sketch of
+ * 2^27 - 1 cells per row, with the reference's second hash (dense seeds: that
+ * of the first k-mer of the current run of valid k-mers; codon-phased: that
+ * of the k-mer itself) and its flat cell layout; exact = 1: counts per 64-bit
+ * hash (HashCounter).  At most 2^29 - 1 windows per sample and k when
+ * counting.  min_count = 0 and codon_phased = 0 is ppsk_sketch.  Replaces
+ * sketch() with codon-phased seeds or a read counter and the GPU count-min
+ * sketcher (src/gpu/sketch.cu:140-176, src/gpu/gpu_countmin.cu).
  */
-int ppsk_sketch_filtered(int device_id, int64_t n_samples, const char *seq,
-                         int64_t seq_bytes, const int64_t *contig_off,
-                         int64_t n_contigs, const int64_t *sample_contig,
-                         int32_t n_k, const int32_t *klist, int32_t sketchsize64,
-                         int32_t use_rc, int32_t min_count, int32_t exact,
-                         uint64_t *raw_out, double *inv_minhash, int32_t *densified);
+int ppsk_sketch_ex(int device_id, int64_t n_samples, const char *seq,
+                   int64_t seq_bytes, const int64_t *contig_off,
+                   int64_t n_contigs, const int64_t *sample_contig, int32_t n_k,
+                   const int32_t *klist, int32_t sketchsize64, int32_t use_rc,
+                   int32_t codon_phased, int32_t min_count, int32_t exact,
+                   uint64_t *raw_out, double *inv_minhash, int32_t *densified);
 
 /*
  * Seeded synthetic sketches (no reference counterpart: a generator for tests

@@ -505,12 +505,12 @@ static uint64_t orc_univhash(uint64_t s, uint64_t t) {
  * (0 none, 1 densified, -1 every bin empty); *inv_minhash = bin 0 / (2^61-1)
  * before densification (sketch.cpp:72-75). */
 int orc_sketch_filtered(const char *seq, const long long *off, int n_contigs, int k, int s64,
-                        int use_rc, int min_count, int exact, uint64_t *usigs,
+                        int use_rc, int codon, int min_count, int exact, uint64_t *usigs,
                         double *inv_minhash);
 
 int orc_sketch(const char *seq, const long long *off, int n_contigs, int k, int s64,
                int use_rc, uint64_t *usigs, double *inv_minhash) {
-  return orc_sketch_filtered(seq, off, n_contigs, k, s64, use_rc, 0, 0, usigs, inv_minhash);
+  return orc_sketch_filtered(seq, off, n_contigs, k, s64, use_rc, 0, 0, 0, usigs, inv_minhash);
 }
 
 /* ---------------------------------------------------------------------------
@@ -566,26 +566,34 @@ struct OrcCounter {
   }
 };
 
+/* codon: the codon-phased seed of generate_seeds (sketch/sketch.cpp:43-54),
+ * 1001001 then 001 repeated, a span of 3k - 2 bases whose every third base
+ * (offsets 0, 3, 6, ...) counts; the hash is the span's ntHash with the
+ * other positions' terms removed (sketch/nthash.hpp:634-722), and every base
+ * of the span must be valid.  The spaced-seed iterator recomputes the extra
+ * hashes at every k-mer (stHashIterator.hpp:96-99). */
 int orc_sketch_filtered(const char *seq, const long long *off, int n_contigs, int k, int s64,
-                        int use_rc, int min_count, int exact, uint64_t *usigs,
+                        int use_rc, int codon, int min_count, int exact, uint64_t *usigs,
                         double *inv_minhash) {
   const uint64_t nbins = (uint64_t)s64 * 64;
   const uint64_t binsize = (ORC_SIGN_MOD + nbins - 1ULL) / nbins;
   std::vector<uint64_t> signs(nbins, UINT64_MAX);
   if (min_count > 255) min_count = 255;  // create_sketches truncates to uint8
-  OrcCounter counter{exact, (unsigned)k, {}};
+  const int span = codon ? 3 * k - 2 : k;
+  OrcCounter counter{exact, (unsigned)span, {}};
   for (int c = 0; c < n_contigs; ++c) {
     const long long a = off[c], len = off[c + 1] - off[c];
     bool prev_ok = false;
     uint64_t run_h1 = 0;
-    for (long long p = 0; p + k <= len; ++p) {
+    for (long long p = 0; p + span <= len; ++p) {
       uint64_t fh = 0, rh = 0;
       bool ok = true;
-      for (int i = 0; i < k; ++i) {
+      for (int i = 0; i < span; ++i) {
         const unsigned char ch = (unsigned char)seq[a + p + i];
         const uint64_t sd = orc_seed(ch);
         if (!sd) { ok = false; break; }
-        fh ^= orc_rot(sd, (unsigned)(k - 1 - i));
+        if (codon && i % 3 != 0) continue;
+        fh ^= orc_rot(sd, (unsigned)(span - 1 - i));
         rh ^= orc_rot(orc_seed_comp(ch), (unsigned)i);
       }
       if (!ok) {
@@ -593,7 +601,7 @@ int orc_sketch_filtered(const char *seq, const long long *off, int n_contigs, in
         continue;
       }
       const uint64_t h0 = (use_rc && rh < fh) ? rh : fh;
-      if (!prev_ok) run_h1 = orc_multi_hash(h0, (unsigned)k, 1);
+      if (!prev_ok || codon) run_h1 = orc_multi_hash(h0, (unsigned)span, 1);
       prev_ok = true;
       if (min_count > 0 && (int)counter.add(h0, run_h1) < min_count) continue;
       const uint64_t h = h0 % ORC_SIGN_MOD;

@@ -62,7 +62,7 @@ def _load():
                                _c.POINTER(_c.c_double)]
     lib.orc_sketch_filtered.restype = _c.c_int
     lib.orc_sketch_filtered.argtypes = [_c.c_char_p, _p, _c.c_int, _c.c_int, _c.c_int,
-                                        _c.c_int, _c.c_int, _c.c_int, _p,
+                                        _c.c_int, _c.c_int, _c.c_int, _c.c_int, _p,
                                         _c.POINTER(_c.c_double)]
     lib.orc_knn_rows.restype = None
     lib.orc_knn_rows.argtypes = [_p, _p, _ll, _ll, _c.c_int, _p, _p, _p, _c.c_int]
@@ -258,9 +258,11 @@ def knn_from_square(square, kNN, num_threads=None):
     return iv, jv, dv
 
 
-def sketch_filtered(contigs, k, sketchsize64, use_rc=True, min_count=0, exact=False):
+def sketch_filtered(contigs, k, sketchsize64, use_rc=True, min_count=0, exact=False,
+                    codon=False):
     """sketch() with the reads count filter (sketch/countmin.cpp): a k-mer
-    occurrence is binned only if its count so far reaches min_count."""
+    occurrence is binned only if its count so far reaches min_count; codon:
+    the codon-phased seeds (sketch/sketch.cpp:43-54)."""
     L = lib()
     bs = [c.encode() if isinstance(c, str) else bytes(c) for c in contigs]
     seq = b"".join(bs)
@@ -269,8 +271,8 @@ def sketch_filtered(contigs, k, sketchsize64, use_rc=True, min_count=0, exact=Fa
     us = np.zeros(sketchsize64 * 14, np.uint64)
     mh = _c.c_double()
     dens = L.orc_sketch_filtered(seq, _ptr(off), len(bs), int(k), int(sketchsize64),
-                                 int(bool(use_rc)), int(min_count), int(bool(exact)), _ptr(us),
-                                 _c.byref(mh))
+                                 int(bool(use_rc)), int(bool(codon)), int(min_count),
+                                 int(bool(exact)), _ptr(us), _c.byref(mh))
     return us, mh.value, dens
 
 
@@ -301,7 +303,7 @@ def ref_sketch_lib():
             return None
         L = _c.CDLL(path)
         L.ref_sketch_seqs.argtypes = [_c.POINTER(_c.c_char_p), _c.c_int, _c.c_int, _c.c_int,
-                                      _c.c_int, _p, _c.POINTER(_c.c_double),
+                                      _c.c_int, _c.c_int, _p, _c.POINTER(_c.c_double),
                                       _c.POINTER(_c.c_int)]
         L.ref_read_stats.argtypes = [_c.c_char_p, _c.c_int, _p, _c.POINTER(_c.c_uint64),
                                      _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_int),
@@ -310,13 +312,13 @@ def ref_sketch_lib():
         L.ref_read_seqs.argtypes = [_c.c_char_p, _c.c_int, _c.c_char_p, _c.c_uint64, _p,
                                     _c.c_uint64]
         L.ref_sketch_file.argtypes = [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int,
-                                      _c.c_int, _c.c_int, _p, _c.POINTER(_c.c_double),
+                                      _c.c_int, _c.c_int, _c.c_int, _p, _c.POINTER(_c.c_double),
                                       _c.POINTER(_c.c_int)]
         _REF_SKETCH = L
     return _REF_SKETCH
 
 
-def ref_sketch(contigs, k, sketchsize64, use_rc=True):
+def ref_sketch(contigs, k, sketchsize64, use_rc=True, codon=False):
     """The reference's compiled sketch() on the same input."""
     L = ref_sketch_lib()
     bs = [c.encode() if isinstance(c, str) else bytes(c) for c in contigs]
@@ -324,8 +326,8 @@ def ref_sketch(contigs, k, sketchsize64, use_rc=True):
     us = np.zeros(sketchsize64 * 14, np.uint64)
     mh = _c.c_double()
     dn = _c.c_int()
-    if L.ref_sketch_seqs(arr, len(bs), int(k), int(sketchsize64), int(bool(use_rc)), _ptr(us),
-                         _c.byref(mh), _c.byref(dn)):
+    if L.ref_sketch_seqs(arr, len(bs), int(k), int(sketchsize64), int(bool(use_rc)),
+                         int(bool(codon)), _ptr(us), _c.byref(mh), _c.byref(dn)):
         raise RuntimeError("reference sketch failed")
     return us, mh.value, dn.value
 
@@ -356,7 +358,8 @@ def ref_read_seqs(path, kmer_len, cap=1 << 26, max_seqs=1 << 20):
     return out
 
 
-def ref_sketch_file(path, keep_len, k, sketchsize64, use_rc=True, min_count=0, exact=False):
+def ref_sketch_file(path, keep_len, k, sketchsize64, use_rc=True, min_count=0, exact=False,
+                    codon=False):
     """The reference's sketch of one file at one k (sequences kept if at
     least keep_len long; reads filtered by count when min_count > 0); None
     when it keeps no sequence."""
@@ -365,8 +368,8 @@ def ref_sketch_file(path, keep_len, k, sketchsize64, use_rc=True, min_count=0, e
     mh = _c.c_double()
     dn = _c.c_int()
     rc = L.ref_sketch_file(path.encode(), int(keep_len), int(k), int(sketchsize64),
-                           int(bool(use_rc)), int(min_count), int(bool(exact)), _ptr(us),
-                           _c.byref(mh), _c.byref(dn))
+                           int(bool(use_rc)), int(bool(codon)), int(min_count), int(bool(exact)),
+                           _ptr(us), _c.byref(mh), _c.byref(dn))
     if rc < 0:
         raise RuntimeError("reference sketch failed")
     if rc == 1:

@@ -13,17 +13,18 @@
 #include "sketch/sketch.hpp"
 
 // Sketch of one sample given as n_seqs in-memory sequences (one per
-// contig), one k-mer length, dense (not codon-phased) seeds.  usigs:
+// contig), one k-mer length, dense or codon-phased seeds.  usigs:
 // [s64 * 14].  Returns 0, or -1 if the reference threw.
 extern "C" int ref_sketch_seqs(const char *const *seqs, int n_seqs, int k, int s64,
-                               int use_rc, uint64_t *usigs, double *inv_minhash,
+                               int use_rc, int codon, uint64_t *usigs, double *inv_minhash,
                                int *densified) {
   try {
     std::vector<std::string> in;
     for (int i = 0; i < n_seqs; ++i) in.emplace_back(seqs[i]);
     SeqBuf buf(in);
-    KmerSeeds seeds = generate_seeds({(size_t)k}, false);
-    auto res = sketch(buf, (uint64_t)s64, seeds[(size_t)k], 14, false, use_rc != 0, 0, false);
+    KmerSeeds seeds = generate_seeds({(size_t)k}, codon != 0);
+    auto res = sketch(buf, (uint64_t)s64, seeds[(size_t)k], 14, codon != 0, use_rc != 0, 0,
+                      false);
     const std::vector<uint64_t> &u = std::get<0>(res);
     std::memcpy(usigs, u.data(), u.size() * sizeof(uint64_t));
     *inv_minhash = std::get<1>(res);
@@ -86,14 +87,14 @@ extern "C" int64_t ref_read_seqs(const char *path, int kmer_len, char *out, uint
 // counter over the count-min sketch).  Returns 1 when the file holds no kept
 // sequence (Reference throws there).
 extern "C" int ref_sketch_file(const char *path, int keep_len, int k, int s64, int use_rc,
-                               int min_count, int exact, uint64_t *usigs, double *inv_minhash,
-                               int *densified) {
+                               int codon, int min_count, int exact, uint64_t *usigs,
+                               double *inv_minhash, int *densified) {
   try {
     SeqBuf buf(std::vector<std::string>{path}, (size_t)keep_len);
     if (buf.nseqs() == 0) return 1;
-    KmerSeeds seeds = generate_seeds({(size_t)k}, false);
+    KmerSeeds seeds = generate_seeds({(size_t)k}, codon != 0);
     if (min_count > 255) min_count = 255;
-    auto res = sketch(buf, (uint64_t)s64, seeds[(size_t)k], 14, false, use_rc != 0,
+    auto res = sketch(buf, (uint64_t)s64, seeds[(size_t)k], 14, codon != 0, use_rc != 0,
                       (uint8_t)min_count, exact != 0);
     const std::vector<uint64_t> &u = std::get<0>(res);
     std::memcpy(usigs, u.data(), u.size() * sizeof(uint64_t));

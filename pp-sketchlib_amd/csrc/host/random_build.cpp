@@ -252,9 +252,6 @@ RandomTables build_random(const std::vector<Sketch>& sketches, unsigned n_cluste
                           bool codon_phased, bool use_rc, int num_threads, int device_id) {
   std::cerr << "Calculating random match chances using Monte Carlo" << std::endl;
   if (sketches.empty()) throw std::runtime_error("No sketches to calculate random matches from");
-  if (codon_phased) {
-    throw std::runtime_error("Codon phased seeds not yet implemented for GPU sketching");
-  }
   if (n_clusters >= sketches.size()) {
     std::cerr << "Cannot make more base frequency clusters than sketches" << std::endl;
     n_clusters = (unsigned)sketches.size() - 1;
@@ -312,9 +309,10 @@ RandomTables build_random(const std::vector<Sketch>& sketches, unsigned n_cluste
   std::vector<int32_t> k32(kl.begin(), kl.end());
   const size_t words = s64 * 14, K = kl.size();
   std::vector<uint64_t> raw(n_rand * K * words);
-  check(ppsk_sketch(device_id, (int64_t)n_rand, seq.data(), (int64_t)seq.size(), off.data(),
-                    (int64_t)n_rand, sc.data(), (int32_t)K, k32.data(), (int32_t)s64,
-                    use_rc ? 1 : 0, raw.data(), nullptr, nullptr));
+  check(ppsk_sketch_ex(device_id, (int64_t)n_rand, seq.data(), (int64_t)seq.size(), off.data(),
+                       (int64_t)n_rand, sc.data(), (int32_t)K, k32.data(), (int32_t)s64,
+                       use_rc ? 1 : 0, codon_phased ? 1 : 0, 0, 0, raw.data(), nullptr,
+                       nullptr));
   std::string().swap(seq);
 
   // pairwise bin-match counts of the random genomes, every k

@@ -159,17 +159,14 @@ bool file_exists(const std::string& p) {
 std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::string>& names,
                                  const std::vector<std::vector<std::string>>& files,
                                  const std::vector<size_t>& kmer_lengths, size_t sketchsize64,
-                                 bool use_rc, size_t min_count, bool exact, size_t num_threads,
-                                 int device_id);
+                                 bool codon_phased, bool use_rc, size_t min_count, bool exact,
+                                 size_t num_threads, int device_id);
 
 void construct_database(const std::string& db_name, const std::vector<std::string>& names,
                         const std::vector<std::vector<std::string>>& files,
                         std::vector<size_t> kmer_lengths, size_t sketchsize64, bool codon_phased,
                         bool calc_random, bool use_rc, size_t min_count, bool exact,
                         size_t num_threads, int device_id) {
-  if (codon_phased) {
-    throw std::runtime_error("Codon phased seeds not yet implemented for GPU sketching");
-  }
   if (names.size() != files.size()) throw std::runtime_error("names and files differ in length");
   std::sort(kmer_lengths.begin(), kmer_lengths.end());
   if (kmer_lengths.empty() || kmer_lengths.front() < 3) {
@@ -184,8 +181,8 @@ void construct_database(const std::string& db_name, const std::vector<std::strin
     resketch = all.size() != names.size();
   }
   if (resketch) {
-    all = sketch_files(h5, names, files, kmer_lengths, sketchsize64, use_rc, min_count, exact,
-                       num_threads, device_id);
+    all = sketch_files(h5, names, files, kmer_lengths, sketchsize64, codon_phased, use_rc,
+                       min_count, exact, num_threads, device_id);
   }
   // constructDatabase (src/sketchlib_bindings.cpp:77-86)
   if (calc_random) {
@@ -201,8 +198,8 @@ void construct_database(const std::string& db_name, const std::vector<std::strin
 std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::string>& names,
                                  const std::vector<std::vector<std::string>>& files,
                                  const std::vector<size_t>& kmer_lengths, size_t sketchsize64,
-                                 bool use_rc, size_t min_count, bool exact, size_t num_threads,
-                                 int device_id) {
+                                 bool codon_phased, bool use_rc, size_t min_count, bool exact,
+                                 size_t num_threads, int device_id) {
   std::cerr << "Sketching " << names.size() << " genomes on GPU device " << device_id << std::endl;
   std::vector<SampleSeq> seqs(names.size());
   std::vector<std::string> errors(names.size());
@@ -229,7 +226,8 @@ std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::s
   all.reserve(names.size());
   std::vector<int32_t> kl(kmer_lengths.begin(), kmer_lengths.end());
   const size_t K = kl.size(), words = sketchsize64 * 14;
-  write_new_db(h5, use_rc, false, SKETCH_VERSION);
+  if (codon_phased) std::cerr << "NB: codon phased seeds are ON" << std::endl;
+  write_new_db(h5, use_rc, codon_phased, SKETCH_VERSION);
   // reads are counted before binning when min_count > 0 (sketch.cpp:128-149;
   // create_sketches truncates min_count to uint8, api.cpp:66-69)
   const int32_t mc = (int32_t)std::min<size_t>(min_count, 255);
@@ -259,16 +257,11 @@ std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::s
     std::vector<uint64_t> raw((size_t)n * K * words);
     std::vector<double> mh(n);
     std::vector<int32_t> dn(n);
-    const int rc =
-        filtered(s0)
-            ? ppsk_sketch_filtered(device_id, n, seq.data(), (int64_t)seq.size(), off.data(),
-                                   (int64_t)off.size() - 1, sc.data(), (int32_t)K, kl.data(),
-                                   (int32_t)sketchsize64, use_rc ? 1 : 0, mc, exact ? 1 : 0,
-                                   raw.data(), mh.data(), dn.data())
-            : ppsk_sketch(device_id, n, seq.data(), (int64_t)seq.size(), off.data(),
-                          (int64_t)off.size() - 1, sc.data(), (int32_t)K, kl.data(),
-                          (int32_t)sketchsize64, use_rc ? 1 : 0, raw.data(), mh.data(),
-                          dn.data());
+    const int rc = ppsk_sketch_ex(device_id, n, seq.data(), (int64_t)seq.size(), off.data(),
+                                  (int64_t)off.size() - 1, sc.data(), (int32_t)K, kl.data(),
+                                  (int32_t)sketchsize64, use_rc ? 1 : 0, codon_phased ? 1 : 0,
+                                  filtered(s0) ? mc : 0, exact ? 1 : 0, raw.data(), mh.data(),
+                                  dn.data());
     if (rc != PPSK_OK) throw std::runtime_error(ppsk_last_error());
     std::vector<Sketch> out;
     for (int64_t x = 0; x < n; ++x) {

@@ -1,4 +1,4 @@
-// Sketching entry points (include/ppsk.h, ppsk_sketch / ppsk_sketch_filtered):
+// Sketching entry points (include/ppsk.h, ppsk_sketch / ppsk_sketch_ex):
 // work units, device buffers and the launches of sketch_kernels.hip.
 #include <hip/hip_runtime.h>
 
@@ -45,8 +45,8 @@ int64_t sample_units(const int64_t* contig_off, int64_t c0, int64_t c1, int64_t 
 int sketch_impl(int device_id, int64_t n_samples, const char* seq, int64_t seq_bytes,
                 const int64_t* contig_off, int64_t n_contigs, const int64_t* sample_contig,
                 int32_t n_k, const int32_t* klist, int32_t sketchsize64, int32_t use_rc,
-                int32_t min_count, int32_t exact, uint64_t* raw_out, double* inv_minhash,
-                int32_t* densified) {
+                int32_t codon, int32_t min_count, int32_t exact, uint64_t* raw_out,
+                double* inv_minhash, int32_t* densified) {
   if (n_samples < 1 || !sample_contig || !raw_out) {
     return fail(PPSK_ERR_ARG, "sketching needs at least one sample and an output buffer");
   }
@@ -58,8 +58,9 @@ int sketch_impl(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
   }
   for (int i = 0; i < n_k; ++i) {
     if (klist[i] < 3) return fail(PPSK_ERR_ARG, "Minimum k must be 3 or higher");
-    if (klist[i] > ppsk::SKETCH_MAX_K) {
-      return fail(PPSK_ERR_ARG, "k must be at most " + std::to_string(ppsk::SKETCH_MAX_K));
+    if ((codon ? 3 * klist[i] : klist[i]) > ppsk::SKETCH_MAX_K) {
+      return fail(PPSK_ERR_ARG, "k must be at most " +
+                                    std::to_string(ppsk::SKETCH_MAX_K / (codon ? 3 : 1)));
     }
     if (i && klist[i] <= klist[i - 1]) return fail(PPSK_ERR_ARG, "klist must be strictly ascending");
   }
@@ -78,12 +79,14 @@ int sketch_impl(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
       return fail(PPSK_ERR_ARG, "contig offsets outside the sequence buffer");
     }
   }
+  // windows span k bases, or 3k - 2 with codon-phased seeds
+  auto span = [codon](int32_t k) -> int64_t { return codon ? 3 * (int64_t)k - 2 : k; };
   const bool filtered = min_count > 0;
   if (filtered) {
     for (int64_t s = 0; s < n_samples; ++s) {
       int64_t w = 0;
       for (int64_t ci = sample_contig[s]; ci < sample_contig[s + 1]; ++ci) {
-        w += std::max<int64_t>(0, contig_off[ci + 1] - contig_off[ci] - klist[0] + 1);
+        w += std::max<int64_t>(0, contig_off[ci + 1] - contig_off[ci] - span(klist[0]) + 1);
       }
       if (w > ppsk::READS_MAX_WINDOWS) {
         return fail(PPSK_ERR_ARG, "a sample has more than " +
@@ -104,7 +107,8 @@ int sketch_impl(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
     for (int kk = 0; kk < n_k; ++kk) {
       for (int64_t s = 0; s < n_samples; ++s) {
         const int64_t u0 = (int64_t)units.size();
-        sample_units(contig_off, sample_contig[s], sample_contig[s + 1], klist[kk], units, nullptr);
+        sample_units(contig_off, sample_contig[s], sample_contig[s + 1], span(klist[kk]), units,
+                     nullptr);
         const int64_t u1 = (int64_t)units.size();
         for (int64_t g = u0; g < u1; g += GROUP_UNITS) {
           groups.push_back({g, std::min(u1, g + GROUP_UNITS), (int32_t)s, kk});
@@ -120,6 +124,7 @@ int sketch_impl(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
   a.n_k = n_k;
   a.s64 = sketchsize64;
   a.use_rc = use_rc ? 1 : 0;
+  a.codon = codon ? 1 : 0;
   a.n_samples = n_samples;
   a.binsize = (sign_mod + (uint64_t)nbins - 1ULL) / (uint64_t)nbins;
   a.bin_recip = ~0ULL / a.binsize;
@@ -180,7 +185,7 @@ int sketch_impl(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
     for (int64_t s = 0; s < n_samples; ++s) {
       std::vector<ppsk::SketchUnit> u;
       const int64_t w = sample_units(contig_off, sample_contig[s], sample_contig[s + 1],
-                                     klist[0], u, nullptr);
+                                     span(klist[0]), u, nullptr);
       max_w = std::max(max_w, w);
       max_u = std::max<int64_t>(max_u, (int64_t)u.size());
     }
@@ -200,7 +205,7 @@ int sketch_impl(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
         jw.clear();
         // the smallest k has the most windows: every later job fits max_w / max_u
         const int64_t nw = sample_units(contig_off, sample_contig[s], sample_contig[s + 1],
-                                        klist[kk], ju, &jw);
+                                        span(klist[kk]), ju, &jw);
         if (nw == 0) continue;
         HIP_TRY(hipMemcpyAsync(d_ju.p, ju.data(), ju.size() * sizeof(ppsk::SketchUnit),
                                hipMemcpyHostToDevice, c->stream));
@@ -216,6 +221,7 @@ int sketch_impl(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
         rj.use_rc = use_rc ? 1 : 0;
         rj.min_count = min_count;
         rj.exact = exact ? 1 : 0;
+        rj.codon = codon ? 1 : 0;
         rj.binsize = a.binsize;
         rj.recip = (uint32_t)a.bin_recip;
         rj.bins = a.signs + (s * n_k + kk) * nbins;
@@ -264,17 +270,17 @@ int ppsk_sketch(int device_id, int64_t n_samples, const char* seq, int64_t seq_b
                 int32_t n_k, const int32_t* klist, int32_t sketchsize64, int32_t use_rc,
                 uint64_t* raw_out, double* inv_minhash, int32_t* densified) {
   return sketch_impl(device_id, n_samples, seq, seq_bytes, contig_off, n_contigs, sample_contig,
-                     n_k, klist, sketchsize64, use_rc, 0, 0, raw_out, inv_minhash, densified);
+                     n_k, klist, sketchsize64, use_rc, 0, 0, 0, raw_out, inv_minhash, densified);
 }
 
-int ppsk_sketch_filtered(int device_id, int64_t n_samples, const char* seq, int64_t seq_bytes,
-                         const int64_t* contig_off, int64_t n_contigs,
-                         const int64_t* sample_contig, int32_t n_k, const int32_t* klist,
-                         int32_t sketchsize64, int32_t use_rc, int32_t min_count, int32_t exact,
-                         uint64_t* raw_out, double* inv_minhash, int32_t* densified) {
+int ppsk_sketch_ex(int device_id, int64_t n_samples, const char* seq, int64_t seq_bytes,
+                   const int64_t* contig_off, int64_t n_contigs, const int64_t* sample_contig,
+                   int32_t n_k, const int32_t* klist, int32_t sketchsize64, int32_t use_rc,
+                   int32_t codon_phased, int32_t min_count, int32_t exact, uint64_t* raw_out,
+                   double* inv_minhash, int32_t* densified) {
   return sketch_impl(device_id, n_samples, seq, seq_bytes, contig_off, n_contigs, sample_contig,
-                     n_k, klist, sketchsize64, use_rc, min_count, exact, raw_out, inv_minhash,
-                     densified);
+                     n_k, klist, sketchsize64, use_rc, codon_phased, min_count, exact, raw_out,
+                     inv_minhash, densified);
 }
 
 }  // extern "C"

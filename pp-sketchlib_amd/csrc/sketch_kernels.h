@@ -9,7 +9,7 @@ namespace ppsk {
 
 constexpr int SKETCH_TB = 512;  // threads per hashing workgroup
 constexpr int BBITS_SK = 14;    // bit layers written per bin
-constexpr int SKETCH_MAX_K = 255;
+constexpr int SKETCH_MAX_K = 255;  // dense k; codon-phased: 3k <= this
 // Base codes (one byte per base, 0 A, 1 C, 2 G, 3 T/U, 4 other) are kept
 // with this many bytes of code 4 before and after the sequence, so that the
 // per-lane 16-byte stream loads never leave the buffer.
@@ -39,6 +39,7 @@ struct SketchArgs {
   int32_t n_k;
   int32_t s64;
   int32_t use_rc;
+  int32_t codon;             // codon-phased seeds (span 3k - 2, every third base)
   int64_t n_samples;
   uint64_t binsize;          // ceil((2^61 - 1) / (64 s64))
   uint64_t bin_recip;        // floor((2^64 - 1) / binsize), < 2^32
@@ -81,6 +82,7 @@ struct ReadsJob {
   int use_rc;
   int min_count;  // 1 .. 255
   int exact;      // hash counter instead of the count-min sketch
+  int codon;      // codon-phased seeds
   uint64_t binsize;
   uint32_t recip;
   uint64_t* bins;  // this (sample, k)'s bin minima

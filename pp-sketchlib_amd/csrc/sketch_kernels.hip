@@ -71,14 +71,19 @@ __device__ __forceinline__ uint64_t seed_of(int code) {
   return code == 0 ? SEED_A : code == 1 ? SEED_C : code == 2 ? SEED_G : code == 3 ? SEED_T : 0;
 }
 
-__device__ __forceinline__ uint64_t rot1(uint64_t x) {
+// rot^S and rot^-S (S < 31)
+template <int S>
+__device__ __forceinline__ uint64_t rot_by(uint64_t x) {
   const uint64_t hi = x >> 33, lo = x & 0x1FFFFFFFFULL;
-  return ((((hi << 1) | (hi >> 30)) & 0x7FFFFFFFULL) << 33) | (((lo << 1) | (lo >> 32)) & 0x1FFFFFFFFULL);
+  return ((((hi << S) | (hi >> (31 - S))) & 0x7FFFFFFFULL) << 33) |
+         (((lo << S) | (lo >> (33 - S))) & 0x1FFFFFFFFULL);
 }
 
-__device__ __forceinline__ uint64_t rot1_inv(uint64_t x) {
+template <int S>
+__device__ __forceinline__ uint64_t rot_inv_by(uint64_t x) {
   const uint64_t hi = x >> 33, lo = x & 0x1FFFFFFFFULL;
-  return ((((hi >> 1) | (hi << 30)) & 0x7FFFFFFFULL) << 33) | (((lo >> 1) | (lo << 32)) & 0x1FFFFFFFFULL);
+  return ((((hi >> S) | (hi << (31 - S))) & 0x7FFFFFFFULL) << 33) |
+         (((lo >> S) | (lo << (33 - S))) & 0x1FFFFFFFFULL);
 }
 
 __device__ __forceinline__ uint64_t rotj(uint64_t x, unsigned j) {
@@ -128,26 +133,40 @@ __global__ void sketch_decode_kernel(uint8_t* codes, int64_t n) {
   for (int64_t x = t0; x < CODE_PAD_BACK; x += stride) codes[n + x] = 4;
 }
 
-// The 25 step terms for k: entry cin * 5 + cout (codes 4 contribute 0).
-__device__ __forceinline__ StepTerms step_terms(int e, int k) {
+// The 25 step terms for an outgoing-base lag: entry cin * 5 + cout (codes 4
+// contribute 0).  The lag is k for dense seeds and 3k for codon-phased ones.
+__device__ __forceinline__ StepTerms step_terms(int e, int lag) {
   const int cin = e / 5, cout = e % 5;
   StepTerms t;
-  t.f = (cin < 4 ? seed_of(cin) : 0ULL) ^ (cout < 4 ? rotj(seed_of(cout), k) : 0ULL);
-  t.r = (cout < 4 ? seed_of(3 - cout) : 0ULL) ^ (cin < 4 ? rotj(seed_of(3 - cin), k) : 0ULL);
+  t.f = (cin < 4 ? seed_of(cin) : 0ULL) ^ (cout < 4 ? rotj(seed_of(cout), lag) : 0ULL);
+  t.r = (cout < 4 ? seed_of(3 - cout) : 0ULL) ^ (cin < 4 ? rotj(seed_of(3 - cin), lag) : 0ULL);
   return t;
 }
 
+// Seed shape: dense (STRIDE 1, span k) or codon-phased (STRIDE 3: bases at
+// offsets 0, 3, ..., 3(k-1) of a span of 3k - 2, sketch/sketch.cpp:43-54).
+// The codon-phased hash is the dense recurrence over every third base, so
+// three interleaved states roll with rot^3, each seeing every third step,
+// and the outgoing base of a state lies 3k bases back.  A window is valid
+// when its whole span is free of code-4 bases.
+template <int STRIDE>
+__device__ __forceinline__ int seed_span(int k) {
+  return STRIDE == 1 ? k : 3 * k - 2;
+}
+
 // One lane rolls through one unit: emit(p, h) for every window p (0-based in
-// the unit, ascending) free of code-4 bases, h its ntHash (the smaller of
-// the two strands with use_rc).  Step i rolls in base i of the unit and,
-// from i = k on, rolls out base i - k; step i ends the window starting at
-// i - k + 1.  Bases come as two streams of 16-byte loads issued a block
-// ahead; each half block's step terms are read from LDS before its emits.
-template <class Emit>
+// the unit, ascending) whose span is free of code-4 bases, h its ntHash (the
+// smaller of the two strands with use_rc).  Step i rolls in base i of the
+// unit and, from i = lag on, rolls out base i - lag; step i ends the window
+// starting at i - span + 1.  Bases come as two streams of 16-byte loads
+// issued a block ahead; each half block's step terms are read from LDS
+// before its emits.
+template <int STRIDE, class Emit>
 __device__ __forceinline__ void roll_unit(const uint8_t* codes, const SketchUnit su, int k,
                                           bool use_rc, const StepTerms* terms, Emit&& emit) {
-  const int steps = su.n + k - 1;
-  const uintptr_t ia = (uintptr_t)(codes + su.pos), oa = ia - (uintptr_t)k;
+  const int span = seed_span<STRIDE>(k), lag = STRIDE * k;
+  const int steps = su.n + span - 1;
+  const uintptr_t ia = (uintptr_t)(codes + su.pos), oa = ia - (uintptr_t)lag;
   const uint32_t* ip = reinterpret_cast<const uint32_t*>(ia & ~(uintptr_t)3);
   const uint32_t* op = reinterpret_cast<const uint32_t*>(oa & ~(uintptr_t)3);
   const uint32_t ish = (uint32_t)(ia & 3), osh = (uint32_t)(oa & 3);
@@ -161,7 +180,11 @@ __device__ __forceinline__ void roll_unit(const uint8_t* codes, const SketchUnit
   }
   ip += 8;
   op += 8;
-  uint64_t fh = 0, rh = 0;
+  // fh[s], rh[s]: the state that step j of the current block updates when
+  // j % STRIDE == s
+  uint64_t fh[STRIDE], rh[STRIDE];
+#pragma unroll
+  for (int q = 0; q < STRIDE; ++q) fh[q] = rh[q] = 0;
   int last_bad = -1;
   for (int i0 = 0; i0 < steps; i0 += 16) {
     uint32_t inn[4], onn[4];
@@ -187,18 +210,31 @@ __device__ __forceinline__ void roll_unit(const uint8_t* codes, const SketchUnit
         const int j = half * 8 + jj;
         const uint32_t cin = (wi[j >> 2] >> (8 * (j & 3))) & 0xFFu;
         uint32_t cout = (wo[j >> 2] >> (8 * (j & 3))) & 0xFFu;
-        cout = i0 + j >= k ? cout : 4u;
+        cout = i0 + j >= lag ? cout : 4u;
         t[jj] = terms[cin * 5 + cout];
         bad |= (cin == 4u ? 1u : 0u) << jj;
       }
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
-        const int i = i0 + half * 8 + jj;
-        fh = rot1(fh) ^ t[jj].f;
-        rh = rot1_inv(rh ^ t[jj].r);
+        const int j = half * 8 + jj, q = j % STRIDE;
+        const int i = i0 + j;
+        fh[q] = rot_by<STRIDE>(fh[q]) ^ t[jj].f;
+        rh[q] = rot_inv_by<STRIDE>(rh[q] ^ t[jj].r);
         last_bad = (bad >> jj) & 1u ? i : last_bad;
-        if (i - k >= last_bad && i < steps) emit(i - k + 1, (use_rc && rh < fh) ? rh : fh);
+        if (i - span >= last_bad && i < steps) {
+          emit(i - span + 1, (use_rc && rh[q] < fh[q]) ? rh[q] : fh[q]);
+        }
       }
+    }
+    if (STRIDE == 3) {
+      // the next block starts 16 steps on: its step 0 updates this block's state 1
+      const uint64_t f0 = fh[0], r0 = rh[0];
+      fh[0] = fh[1 % STRIDE];
+      rh[0] = rh[1 % STRIDE];
+      fh[1 % STRIDE] = fh[2 % STRIDE];
+      rh[1 % STRIDE] = rh[2 % STRIDE];
+      fh[2 % STRIDE] = f0;
+      rh[2 % STRIDE] = r0;
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -218,7 +254,7 @@ __device__ __forceinline__ uint64_t bin_of(uint64_t h, uint64_t binsize, uint32_
   return bin;
 }
 
-template <bool LDS_BINS>
+template <bool LDS_BINS, int STRIDE>
 __global__ __launch_bounds__(SKETCH_TB) void sketch_hash_kernel(SketchArgs a, int64_t n_groups,
                                                                 int* next_group) {
   extern __shared__ uint64_t lbins[];
@@ -242,14 +278,14 @@ __global__ __launch_bounds__(SKETCH_TB) void sketch_hash_kernel(SketchArgs a, in
     const int kk = g.kk;
     const int k = a.klist[kk];
     uint64_t* gbins = a.signs + ((int64_t)g.sample * a.n_k + kk) * nbins;
-    if (k != cur_k && threadIdx.x < 25) terms[threadIdx.x] = step_terms(threadIdx.x, k);
+    if (k != cur_k && threadIdx.x < 25) terms[threadIdx.x] = step_terms(threadIdx.x, STRIDE * k);
     if (LDS_BINS) {
       for (int64_t b = threadIdx.x; b < nbins; b += SKETCH_TB) lbins[b] = ~0ULL;
     }
     __syncthreads();
     if (threadIdx.x == 0) cur_k = k;
     for (int64_t u = g.unit0 + threadIdx.x; u < g.unit1; u += SKETCH_TB) {
-      roll_unit(a.codes, a.units[u], k, use_rc, terms, [&](int, uint64_t h0) {
+      roll_unit<STRIDE>(a.codes, a.units[u], k, use_rc, terms, [&](int, uint64_t h0) {
         const uint64_t h = mod_sign(h0);
         const uint64_t bin = bin_of(h, binsize, recip);
         if (LDS_BINS) {
@@ -298,16 +334,17 @@ __device__ __forceinline__ uint64_t multi_hash1(uint64_t h0, int k) {
   return t ^ (t >> 27);
 }
 
+template <int STRIDE>
 __global__ __launch_bounds__(256) void reads_hash_kernel(ReadsJob j, uint64_t* h0, uint8_t* st) {
   __shared__ StepTerms terms[25];
-  if (threadIdx.x < 25) terms[threadIdx.x] = step_terms(threadIdx.x, j.k);
+  if (threadIdx.x < 25) terms[threadIdx.x] = step_terms(threadIdx.x, STRIDE * j.k);
   __syncthreads();
   const bool use_rc = j.use_rc != 0;
   for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < j.n_units;
        u += (int64_t)gridDim.x * blockDim.x) {
     const SketchUnit su = j.units[u];
     const int64_t w0 = j.wbase[u];
-    roll_unit(j.codes, su, j.k, use_rc, terms, [&](int p, uint64_t h) {
+    roll_unit<STRIDE>(j.codes, su, j.k, use_rc, terms, [&](int p, uint64_t h) {
       h0[w0 + p] = h;
       st[w0 + p] = (p == 0 && su.pad) ? 3 : 1;
     });
@@ -323,16 +360,19 @@ __global__ void reads_run_kernel(const uint8_t* st, int64_t n, int64_t* rs) {
   }
 }
 
-// occurrence t = tix[w] of every valid window: H0[t], H1[t]
+// occurrence t = tix[w] of every valid window: H0[t], H1[t].  Hash 1 mixes
+// hash 0 of the run's first window (dense seeds) or of the window itself
+// (codon-phased seeds: the spaced-seed iterator recomputes it every step);
+// span is the iterator's k, the seed's span.
 __global__ void reads_compact_kernel(const uint8_t* st, const uint64_t* h0, const int64_t* run,
-                                     const int64_t* tix, int64_t n, int k, uint64_t* H0,
-                                     uint64_t* H1) {
+                                     const int64_t* tix, int64_t n, int span, int own_hash,
+                                     uint64_t* H0, uint64_t* H1) {
   for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < n;
        w += (int64_t)gridDim.x * blockDim.x) {
     if (st[w] != 0) {
       const int64_t t = tix[w];
       H0[t] = h0[w];
-      H1[t] = multi_hash1(h0[run[w]], k);
+      H1[t] = multi_hash1(h0[own_hash ? w : run[w]], span);
     }
   }
 }
@@ -464,7 +504,11 @@ hipError_t run_reads_job(const ReadsJob& j, void* ws, size_t ws_bytes, hipStream
 
   hipError_t e = hipMemsetAsync(st, 0, (size_t)W, stream);
   if (e != hipSuccess) return e;
-  reads_hash_kernel<<<grid_for(j.n_units), 256, 0, stream>>>(j, h0, st);
+  if (j.codon) {
+    reads_hash_kernel<3><<<grid_for(j.n_units), 256, 0, stream>>>(j, h0, st);
+  } else {
+    reads_hash_kernel<1><<<grid_for(j.n_units), 256, 0, stream>>>(j, h0, st);
+  }
   reads_run_kernel<<<grid_for(W), 256, 0, stream>>>(st, W, rs);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   size_t tb = temp_bytes;
@@ -474,7 +518,9 @@ hipError_t run_reads_job(const ReadsJob& j, void* ws, size_t ws_bytes, hipStream
   tb = temp_bytes;
   e = hipcub::DeviceScan::ExclusiveSum(temp, tb, valid, tix, (int)W, stream);
   if (e != hipSuccess) return e;
-  reads_compact_kernel<<<grid_for(W), 256, 0, stream>>>(st, h0, run, tix, W, j.k, H0, H1);
+  const int span = j.codon ? 3 * j.k - 2 : j.k;
+  reads_compact_kernel<<<grid_for(W), 256, 0, stream>>>(st, h0, run, tix, W, span, j.codon, H0,
+                                                        H1);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // occurrence count
   int64_t last_t = 0;
@@ -575,6 +621,24 @@ hipError_t launch_sketch_decode(uint8_t* codes, int64_t n, hipStream_t stream) {
   return hipGetLastError();
 }
 
+template <bool LDS_BINS, int STRIDE>
+hipError_t launch_hash(const SketchArgs& a, int64_t n_groups, size_t lds, int* next_group,
+                       hipStream_t stream) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sketch_hash_kernel<LDS_BINS, STRIDE>,
+                                                     SKETCH_TB, lds);
+  }
+  if (e != hipSuccess) return e;
+  const int64_t grid = std::max<int64_t>(
+      1, std::min<int64_t>(n_groups, (int64_t)std::max(per_cu, 1) * std::max(cus, 1)));
+  sketch_hash_kernel<LDS_BINS, STRIDE><<<(unsigned)grid, SKETCH_TB, lds, stream>>>(a, n_groups,
+                                                                                 next_group);
+  return hipGetLastError();
+}
+
 hipError_t launch_sketch(const SketchArgs& a, int64_t n_groups, bool lds_bins, int* next_group,
                          hipStream_t stream) {
   const int64_t nbins = (int64_t)a.s64 * 64;
@@ -586,25 +650,14 @@ hipError_t launch_sketch(const SketchArgs& a, int64_t n_groups, bool lds_bins, i
   if (n_groups > 0) {
     e = hipMemsetAsync(next_group, 0, sizeof(int), stream);
     if (e != hipSuccess) return e;
-    int dev = 0, cus = 0, per_cu = 0;
-    e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const size_t lds = lds_bins ? (size_t)nbins * sizeof(uint64_t) : 0;
-    if (e == hipSuccess) {
-      e = lds_bins ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sketch_hash_kernel<true>,
-                                                                  SKETCH_TB, lds)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         &per_cu, sketch_hash_kernel<false>, SKETCH_TB, 0);
-    }
-    if (e != hipSuccess) return e;
-    const int64_t grid = std::max<int64_t>(
-        1, std::min<int64_t>(n_groups, (int64_t)std::max(per_cu, 1) * std::max(cus, 1)));
     if (lds_bins) {
-      sketch_hash_kernel<true><<<(unsigned)grid, SKETCH_TB, lds, stream>>>(a, n_groups, next_group);
+      e = a.codon ? launch_hash<true, 3>(a, n_groups, lds, next_group, stream)
+                  : launch_hash<true, 1>(a, n_groups, lds, next_group, stream);
     } else {
-      sketch_hash_kernel<false><<<(unsigned)grid, SKETCH_TB, 0, stream>>>(a, n_groups, next_group);
+      e = a.codon ? launch_hash<false, 3>(a, n_groups, lds, next_group, stream)
+                  : launch_hash<false, 1>(a, n_groups, lds, next_group, stream);
     }
-    e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   sketch_finish_kernel<<<(unsigned)(a.n_samples * a.n_k), 64, 0, stream>>>(a);

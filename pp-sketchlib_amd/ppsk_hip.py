@@ -32,7 +32,7 @@ EXPORTS = [
     "ppsk_last_timing", "ppsk_synth_sketches", "ppsk_set_prefix", "ppsk_batch_create",
     "ppsk_batch_run", "ppsk_batch_tuning", "ppsk_batch_free", "ppsk_server_create",
     "ppsk_server_submit", "ppsk_server_wait", "ppsk_result_free", "ppsk_server_stats",
-    "ppsk_server_free", "ppsk_sketch", "ppsk_sketch_filtered",
+    "ppsk_server_free", "ppsk_sketch", "ppsk_sketch_ex",
 ]
 
 
@@ -116,8 +116,8 @@ def lib():
     L.ppsk_server_free.argtypes = [_p]
     L.ppsk_sketch.argtypes = [_c.c_int, _i64, _c.c_char_p, _i64, _p, _i64, _p, _i32, _p, _i32,
                               _i32, _p, _p, _p]
-    L.ppsk_sketch_filtered.argtypes = [_c.c_int, _i64, _c.c_char_p, _i64, _p, _i64, _p, _i32,
-                                       _p, _i32, _i32, _i32, _i32, _p, _p, _p]
+    L.ppsk_sketch_ex.argtypes = [_c.c_int, _i64, _c.c_char_p, _i64, _p, _i64, _p, _i32, _p,
+                                 _i32, _i32, _i32, _i32, _i32, _p, _p, _p]
     _lib = L
     return L
 
@@ -304,11 +304,12 @@ class Batch:
         self.free()
 
 
-def sketch(samples, klist, sketchsize64, use_rc=True, device=0, min_count=0, exact=False):
+def sketch(samples, klist, sketchsize64, use_rc=True, device=0, min_count=0, exact=False,
+           codon=False):
     """Sketch sequences on the GPU.  samples: list of samples, each a list of
-    contigs (str or bytes); min_count > 0 applies the reads count filter
-    (ppsk_sketch_filtered).  Returns (raw [n][n_k][s64*14] uint64,
-    inv_minhash_sum [n], densified [n])."""
+    contigs (str or bytes); min_count > 0 applies the reads count filter and
+    codon the codon-phased seeds (ppsk_sketch_ex).  Returns (raw
+    [n][n_k][s64*14] uint64, inv_minhash_sum [n], densified [n])."""
     contigs = []
     sample_contig = [0]
     for smp in samples:
@@ -324,11 +325,11 @@ def sketch(samples, klist, sketchsize64, use_rc=True, device=0, min_count=0, exa
     raw = np.zeros((n, len(kl), sketchsize64 * 14), np.uint64)
     mh = np.zeros(n, np.float64)
     dn = np.zeros(n, np.int32)
-    if min_count > 0:
-        _check(lib().ppsk_sketch_filtered(device, n, seq, len(seq), _ptr(off), len(contigs),
-                                          _ptr(sc), len(kl), _ptr(kl), sketchsize64,
-                                          int(bool(use_rc)), int(min_count), int(bool(exact)),
-                                          _ptr(raw), _ptr(mh), _ptr(dn)))
+    if min_count > 0 or codon:
+        _check(lib().ppsk_sketch_ex(device, n, seq, len(seq), _ptr(off), len(contigs), _ptr(sc),
+                                    len(kl), _ptr(kl), sketchsize64, int(bool(use_rc)),
+                                    int(bool(codon)), int(min_count), int(bool(exact)),
+                                    _ptr(raw), _ptr(mh), _ptr(dn)))
     else:
         _check(lib().ppsk_sketch(device, n, seq, len(seq), _ptr(off), len(contigs), _ptr(sc),
                                  len(kl), _ptr(kl), sketchsize64, int(bool(use_rc)), _ptr(raw),

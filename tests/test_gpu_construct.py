@@ -105,6 +105,26 @@ def test_construct_database_errors(pp, tmp_path):
     empty = _put(tmp_path / "empty.fa", ">x\nACG\n")
     with pytest.raises(RuntimeError, match="Errors during sketching"):
         pp.constructDatabase(str(tmp_path / "e"), ["x"], [[empty]], KL, S64)
-    fq = _put(tmp_path / "r.fq", "@r\n" + "ACGT" * 20 + "\n+\n" + "I" * 80 + "\n")
-    with pytest.raises(RuntimeError, match="Codon phased"):
-        pp.constructDatabase(str(tmp_path / "c"), ["r"], [[fq]], KL, S64, codon_phased=True)
+    with pytest.raises(RuntimeError, match="Minimum k"):
+        pp.constructDatabase(str(tmp_path / "k"), ["x"], [[empty]], [2, 15], S64)
+
+
+def test_construct_database_codon_phased(pp, tmp_path):
+    names, files = _samples(tmp_path, seed=12)
+    names, files = names[:3], files[:3]
+    db = str(tmp_path / "codon")
+    kl = [9, 11, 13]
+    pp.constructDatabase(db, names, files, kl, S64, codon_phased=True, calc_random=True)
+    have_ref = O.ref_sketch_lib() is not None
+    for name, fl in zip(names, files):
+        info = pp.getSketch(db, name)
+        _, _, _, _, contigs = pp._readSequences(fl, kl[-1])
+        for k in kl:
+            us, _, _ = O.sketch_filtered(contigs, k, S64, True, codon=True)
+            assert np.array_equal(info["sketch"][k], us), (name, k)
+            if have_ref and len(fl) == 1:
+                ref = O.ref_sketch_file(fl[0], kl[-1], k, S64, True, codon=True)
+                assert np.array_equal(info["sketch"][k], ref[0]), (name, k)
+    assert pp._readRandom(db)["present"]
+    d = pp.queryDatabase(db, db, names, names, kl)
+    assert d.shape == (3, 2) and np.isfinite(d).all()

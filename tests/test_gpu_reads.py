@@ -1,4 +1,4 @@
-"""Read sketching with the count filter on the GPU (ppsk_sketch_filtered,
+"""Read sketching with the count filter on the GPU (ppsk_sketch_ex,
 constructDatabase with min_count > 0) against the oracle's restatement, which
 is itself pinned to the reference's compiled sketch() over FASTQ files
 (tests/test_reads_filter.py).
@@ -95,3 +95,17 @@ def test_construct_database_from_reads(pp, tmp_path):
         assert not asm["reads"]
         for k in kl:
             assert np.array_equal(asm["sketch"][k], O.sketch([genome], k, 16, True)[0])
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_codon_phased_filtered_sketch(exact):
+    # spaced seeds: the count-min sketch's second hash is the k-mer's own
+    samples = [_reads(7, 20000, 1500, 0.01), _reads(8, 15000, 1200, 0.0)]
+    raw, mh, dn = H.sketch(samples, [9, 13], 16, min_count=2, exact=exact, codon=True)
+    for s, contigs in enumerate(samples):
+        want_mh = 0.0
+        for x, k in enumerate([9, 13]):
+            us, m, _ = O.sketch_filtered(contigs, k, 16, True, 2, exact, codon=True)
+            assert np.array_equal(raw[s, x], us), (s, k)
+            want_mh += m
+        assert mh[s] == want_mh

@@ -74,3 +74,36 @@ def test_sketch_tree_genomes_distance_sanity():
     assert nf == 0
     core = d[:3, 0]  # (0,1), (0,2), (0,3)
     assert core[0] < core[1] < core[2]
+
+
+def _check_codon(samples, klist, s64, use_rc=True):
+    raw, mh, dn = H.sketch(samples, klist, s64, use_rc=use_rc, codon=True)
+    for s, contigs in enumerate(samples):
+        want_mh, want_dn = 0.0, 0
+        for x, k in enumerate(klist):
+            us, m, d = O.sketch_filtered(contigs, k, s64, use_rc, codon=True)
+            assert np.array_equal(raw[s, x], us), (s, k)
+            want_mh += m
+            want_dn |= int(d != 0)
+        assert mh[s] == pytest.approx(want_mh, rel=0, abs=1e-15), s
+        assert dn[s] == want_dn, s
+
+
+@pytest.mark.parametrize("use_rc", [True, False])
+def test_codon_phased_sketch(use_rc):
+    # codon-phased seeds: a span of 3k - 2 bases, every third one hashed;
+    # three interleaved rolling states per lane
+    rng = np.random.default_rng(21)
+    samples = [
+        [_rseq(rng, 30000)],
+        [_rseq(rng, 6000, "ACGTacgtNnU"), _rseq(rng, 30), _rseq(rng, 9000)],
+        [_rseq(rng, 1024 + 40), _rseq(rng, 2048 + 43), _rseq(rng, 1023 + 45)],  # unit edges
+        [_rseq(rng, 50)],  # shorter than every span of k >= 17
+    ]
+    _check_codon(samples, [5, 9, 15, 17, 21], 16, use_rc)
+
+
+def test_codon_phased_large_k_and_sketch():
+    rng = np.random.default_rng(22)
+    samples = [[_rseq(rng, 80000)], [_rseq(rng, 40000, "ACGTN")]]
+    _check_codon(samples, [29, 41, 85], 156)

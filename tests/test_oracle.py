@@ -360,6 +360,53 @@ def test_sketch_matches_reference_core(case, rc):
     assert (dens != 0) == bool(rdens)
 
 
+CODON_CASES = [
+    # contigs, k (span 3k - 2), s64
+    ((("ACGT", 6000),), 3, 16),                       # span 7: every bin densified
+    ((("ACGTacgtNnU", 9000), ("ACGT", 50)), 9, 16),  # lowercase, N, U; a short contig
+    ((("ACGT", 20000),), 15, 32),                      # span 43 > 33
+    ((("ACGT", 40000),), 21, 156),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CODON_CASES)))
+@pytest.mark.parametrize("rc", [True, False])
+def test_codon_phased_sketch_matches_reference_core(case, rc):
+    if O.ref_sketch_lib() is None:
+        pytest.skip("reference sketch core not built (oracle/_ref)")
+    spec, k, s64 = CODON_CASES[case]
+    rng = np.random.default_rng(100 + case)
+    contigs = [_rseq(rng, n, alpha) for alpha, n in spec]
+    us, mh, dens = O.sketch_filtered(contigs, k, s64, rc, codon=True)
+    rus, rmh, rdens = O.ref_sketch(contigs, k, s64, rc, codon=True)
+    assert np.array_equal(us, rus)
+    assert mh == rmh
+    assert (dens != 0) == bool(rdens)
+
+
+def test_codon_phased_ignores_the_wobble_positions():
+    # one span-long sequence holds one k-mer: changing bases off the seed
+    # (offsets not divisible by 3) keeps the sketch, changing a seed base
+    # moves it; an N anywhere in the span removes the k-mer
+    rng = np.random.default_rng(8)
+    k, s64 = 5, 1
+    span = 3 * k - 2
+    s = _rseq(rng, span)
+    base, _, _ = O.sketch_filtered([s], k, s64, False, codon=True)
+    wob = list(s)
+    for i in range(span):
+        if i % 3:
+            wob[i] = "ACGT"[("ACGT".index(wob[i]) + 1) % 4]
+    same, _, _ = O.sketch_filtered(["".join(wob)], k, s64, False, codon=True)
+    assert np.array_equal(base, same)
+    seed = list(s)
+    seed[3] = "ACGT"[("ACGT".index(seed[3]) + 1) % 4]
+    moved, _, _ = O.sketch_filtered(["".join(seed)], k, s64, False, codon=True)
+    assert not np.array_equal(base, moved)
+    _, _, dens = O.sketch_filtered([s[:4] + "N" + s[5:]], k, s64, False, codon=True)
+    assert dens == -1  # no k-mer: every bin empty
+
+
 def test_sketch_canonical_strand_symmetry():
     # canonical k-mers: a sequence and its reverse complement sketch alike;
     # forward-only hashing does not

@@ -58,3 +58,14 @@ def test_min_count_zero_is_the_plain_sketch(reads_file, pp):
     a = O.sketch(contigs, 15, 16, True)
     b = O.sketch_filtered(contigs, 15, 16, True, 0, False)
     assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_codon_phased_filtered_sketch_matches_reference(pp, reads_file, exact):
+    # spaced seeds recompute the second hash at every k-mer
+    _need_ref()
+    k = 9  # span 25
+    _, _, _, _, contigs = pp._readSequences([reads_file], k)
+    us, mh, _ = O.sketch_filtered(contigs, k, 32, True, 2, exact, codon=True)
+    rus, rmh, _ = O.ref_sketch_file(reads_file, k, k, 32, True, 2, exact, codon=True)
+    assert np.array_equal(us, rus) and mh == rmh
