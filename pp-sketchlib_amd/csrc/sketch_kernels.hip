@@ -398,13 +398,14 @@ __global__ void reads_iota_kernel(uint32_t* val, int64_t n) {
 
 // after the stable sort: the count of element s's key up to and including
 // it is (rank among equal keys) + 1; it reaches m iff the key m - 1 places
-// back is the same
+// back is the same.  pass starts all 1 and only the (few) failing events are
+// scattered back to their occurrence order.
 template <class Key>
 __global__ void reads_pass_kernel(const Key* key, const uint32_t* val, int64_t n, int m,
                                   uint8_t* pass) {
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
        s += (int64_t)gridDim.x * blockDim.x) {
-    pass[val[s]] = (s >= m - 1 && key[s - (m - 1)] == key[s]) ? 1 : 0;
+    if (!(s >= m - 1 && key[s - (m - 1)] == key[s])) pass[val[s]] = 0;
   }
 }
 
@@ -546,6 +547,8 @@ hipError_t run_reads_job(const ReadsJob& j, void* ws, size_t ws_bytes, hipStream
     e = hipcub::DeviceRadixSort::SortPairs(temp, tb, k64in, k64out, vin, vout, (int)N, 0, 64,
                                            stream);
     if (e != hipSuccess) return e;
+    e = hipMemsetAsync(pass, 1, (size_t)N, stream);
+    if (e != hipSuccess) return e;
     reads_pass_kernel<uint64_t><<<grid_for(N), 256, 0, stream>>>(k64out, vout, N, j.min_count,
                                                                   pass);
   } else {
@@ -554,6 +557,8 @@ hipError_t run_reads_job(const ReadsJob& j, void* ws, size_t ws_bytes, hipStream
     tb = temp_bytes;
     e = hipcub::DeviceRadixSort::SortPairs(temp, tb, kin, kout, vin, vout, (int)(4 * N), 0, 29,
                                            stream);
+    if (e != hipSuccess) return e;
+    e = hipMemsetAsync(pass, 1, (size_t)(4 * N), stream);
     if (e != hipSuccess) return e;
     reads_pass_kernel<uint32_t><<<grid_for(4 * N), 256, 0, stream>>>(kout, vout, 4 * N,
                                                                       j.min_count, pass);
