@@ -1,7 +1,8 @@
-// Host orchestration of the distance path on top of the C ABI (libppsk):
-// the roles of query_db / query_db_sparse / query_db_cuda /
-// query_db_sparse_cuda (src/api.cpp:140-432, src/gpu/gpu_api.cpp:352-531).
-// Every distance is computed by the HIP kernels; there is no CPU path.
+// Host orchestration on top of the C ABI (libppsk): the roles of query_db /
+// query_db_sparse / query_db_cuda / query_db_sparse_cuda (src/api.cpp:140-432,
+// src/gpu/gpu_api.cpp:352-531), of create_sketches (src/api.cpp:39-138) and of
+// the RandomMC constructor.  Every distance and sketch is computed by the HIP
+// kernels; there is no CPU path.
 #pragma once
 
 #include <cstddef>

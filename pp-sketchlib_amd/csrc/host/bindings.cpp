@@ -1,9 +1,9 @@
-// pp_sketchlib: Python module with the reference's API for the distance path
+// pp_sketchlib: Python module with the reference's API
 // (src/sketchlib_bindings.cpp:238-312): same function names, keyword
-// arguments, defaults, return types and error messages.  Distances are
-// computed by the HIP library (libppsk.so); `use_gpu` and `device_id` are
-// accepted as in the reference, and a call that the reference rejects on its
-// GPU path is rejected here in the same way.
+// arguments, defaults, return types and error messages.  Distances and
+// sketches are computed by the HIP library (libppsk.so); `use_gpu` and
+// `device_id` are accepted as in the reference, and a distance call that the
+// reference rejects on its GPU path is rejected here in the same way.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>

@@ -1,7 +1,7 @@
 // Random-match model on the host side: which correction applies, per-sample
 // cluster ids and the table handed to the device.  Mirrors the lookup side of
-// RandomMC (src/random/random_match.hpp:27-117); building Monte Carlo tables
-// (k-means + random genome sketching) is not part of this library.
+// RandomMC (src/random/random_match.hpp:27-117); the tables themselves are
+// built by build_random (random_build.cpp).
 #pragma once
 
 #include <cstddef>
