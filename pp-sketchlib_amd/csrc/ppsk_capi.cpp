@@ -333,13 +333,32 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
     chunks.emplace_back(gs[g], gs[h]);
     g = h;
   }
+  // A chunk's last, partial round of whole tiles would leave most workgroup
+  // slots idle while a few run a whole tile each: when it fills at most half
+  // of the slots, those tiles go to a second launch as (tile, k-mer length)
+  // items of the per-(tile, k) kernel instead (PPSK_TAIL_SPLIT=0 turns this
+  // off).  Same tile code, same epilogue, same output.
+  const char* split_env = std::getenv("PPSK_TAIL_SPLIT");
+  const bool split_tail = !split_env || std::atoi(split_env) != 0;
+  std::vector<size_t> tail(chunks.size(), 0);
+  std::vector<ppsk::Tile> tail_tiles;
+  std::vector<size_t> tail_at(chunks.size(), 0);
+  size_t most_tail = 0;
+  for (size_t ci = 0; ci < chunks.size(); ++ci) {
+    const size_t n = chunks[ci].second - chunks[ci].first, r = n % (size_t)grid;
+    if (split_tail && n > (size_t)grid && r > 0 && r <= (size_t)grid / 2) tail[ci] = r;
+    tail_at[ci] = tail_tiles.size();
+    tail_tiles.insert(tail_tiles.end(), tiles.begin() + (chunks[ci].second - tail[ci]),
+                      tiles.begin() + chunks[ci].second);
+    most_tail = std::max(most_tail, tail[ci]);
+  }
   // items: chunk by chunk, each chunk's tiles cut into one contiguous run per
   // queue (so each XCD walks a compact block)
   std::vector<ppsk::BatchItem> items;
   items.reserve(tiles.size());
   std::vector<std::array<int64_t, ppsk::BATCH_QUEUES + 1>> qb(chunks.size());
   for (size_t ci = 0; ci < chunks.size(); ++ci) {
-    const size_t a = chunks[ci].first, n = chunks[ci].second - a;
+    const size_t a = chunks[ci].first, n = chunks[ci].second - a - tail[ci];
     for (int q = 0; q < ppsk::BATCH_QUEUES; ++q) {
       qb[ci][q] = (int64_t)items.size();
       for (size_t x = a + n * q / ppsk::BATCH_QUEUES; x < a + n * (q + 1) / ppsk::BATCH_QUEUES; ++x) {
@@ -354,6 +373,20 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
   if (!c->pparams) HIP_TRY(hipMalloc(&c->pparams, sizeof(ppsk::DistParams)));
   if (!c->pqueue) HIP_TRY(hipMalloc(&c->pqueue, ppsk::BATCH_QUEUES * sizeof(uint32_t)));
   p.out_row_offset = 0;
+  if (most_tail > 0) {
+    const size_t per_tile = (size_t)p.n_k * tr * tc * sizeof(uint16_t);
+    if (int rc = grow(&c->counts, &c->counts_bytes, most_tail * per_tile)) return rc;
+    if (int rc = grow_tile_done(c, most_tail)) return rc;
+    void* tptr = c->tiles;
+    size_t tbytes = c->tiles_n * sizeof(ppsk::Tile);
+    if (int rc = grow(&tptr, &tbytes, tail_tiles.size() * sizeof(ppsk::Tile))) return rc;
+    c->tiles = (ppsk::Tile*)tptr;
+    c->tiles_n = tbytes / sizeof(ppsk::Tile);
+    HIP_TRY(hipMemcpyAsync(c->tiles, tail_tiles.data(), tail_tiles.size() * sizeof(ppsk::Tile),
+                           hipMemcpyHostToDevice, c->stream));
+    p.counts = (uint16_t*)c->counts;
+    p.tile_done = c->tile_done;
+  }
   HIP_TRY(hipMemcpyAsync(c->pitems, items.data(), items.size() * sizeof(ppsk::BatchItem),
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(c->pparams, &p, sizeof(p), hipMemcpyHostToDevice, c->stream));
@@ -371,8 +404,16 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
     HIP_TRY(ppsk::launch_batch_reset(c->pqueue, nullptr, 0, c->stream));
     HIP_TRY(hipEventRecord(e0, c->stream));
     HIP_TRY(ppsk::launch_batch(args, variant, grid, c->stream));
-    HIP_TRY(hipEventRecord(e1, c->stream));
     tm->launches += 1;
+    if (tail[ci] > 0) {
+      ppsk::DistParams pt = p;
+      pt.tiles = c->tiles + tail_at[ci];
+      pt.n_tiles = (int64_t)tail[ci];
+      pt.item_base = 0;
+      HIP_TRY(ppsk::launch_dists(pt, variant, (int64_t)tail[ci] * p.n_k, c->stream));
+      tm->launches += 1;
+    }
+    HIP_TRY(hipEventRecord(e1, c->stream));
     tm->items += (int64_t)(chunks[ci].second - chunks[ci].first) * p.n_k;
   }
   for (size_t ci = 0; host_out && ci < chunks.size(); ++ci) {

@@ -221,3 +221,30 @@ def test_row_prefetch_off_is_identical(monkeypatch):
         monkeypatch.setenv("PPSK_ROW_PREFETCH", "0")
         off, _, _ = H.dense(s, random_mode=H.RANDOM_EQUAL_BASE)
     assert np.array_equal(on.view(np.uint32), off.view(np.uint32))
+
+
+@pytest.mark.parametrize("variant", [2, 4])
+def test_tail_split_persistent_pass(monkeypatch, variant):
+    # a pass just over one round of whole tiles: its partial last round runs
+    # as (tile, k) items of the per-(tile, k) kernel; outputs are identical to
+    # the whole-tile run and to the oracle
+    monkeypatch.setenv("PPSK_TILE_VARIANT", str(variant))
+    monkeypatch.setenv("PPSK_PERSISTENT", "1")
+    n = 4000 if variant == 2 else 2900
+    raw, le, _ = synth(n)
+    with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
+        c, _, _ = H.dense(s, out="counts")
+        launches = H.last_timing()[1]
+        d, nf, _ = H.dense(s, random_mode=H.RANDOM_EQUAL_BASE)
+        monkeypatch.setenv("PPSK_TAIL_SPLIT", "0")
+        c0, _, _ = H.dense(s, out="counts")
+        launches0 = H.last_timing()[1]
+        d0, _, _ = H.dense(s, random_mode=H.RANDOM_EQUAL_BASE)
+    assert launches == launches0 + 1, (launches, launches0)  # the tail launch ran
+    assert nf == 0
+    assert np.array_equal(c, c0)
+    assert np.array_equal(d.view(np.uint32), d0.view(np.uint32))
+    ref, ref_c, _, _ = O.query_db(raw, KLIST8, 156, 14, le, random=O.Random(1, True),
+                                  want_counts=True)
+    assert np.array_equal(c, ref_c)
+    assert np.abs(d - ref).max() <= CORE_ACC_TOL
