@@ -58,6 +58,10 @@ def parse():
     ap.add_argument("--serve-qps", type=float, default=0.0,
                     help="Poisson arrival rate (0: 70%% of the measured saturation rate)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-max-rows", type=int, default=128,
+                    help="query rows per serving step (larger requests are split)")
+    ap.add_argument("--serve-slots", type=int, default=2,
+                    help="serving steps in flight (worker streams)")
     ap.add_argument("--batch-jobs", type=int, default=256)
     return ap.parse_args()
 

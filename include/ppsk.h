@@ -240,12 +240,17 @@ int ppsk_batch_free(ppsk_batch *batch);
  * reference's closest interface is queryDatabase followed by a sparse
  * extraction, one call per query batch (src/sketchlib_bindings.cpp:89-183).
  *
- * Continuous batching: n_slots workers, each with its own stream, take as
- * many queued requests (oldest first) as fit one step of at most max_rows
- * query samples whose device memory fits the serving arena (arena_bytes;
- * 0 = free HBM less 4 GiB); requests arriving while other steps run join the
- * next step that has room, and steps of different workers overlap on the
- * GPU.  A request's results do not depend on the step it joins.
+ * Continuous batching: n_slots workers, each with its own stream, fill steps
+ * of at most max_rows query samples with queued requests, oldest first, while
+ * the step's device memory fits the serving arena (arena_bytes; 0 = free HBM
+ * less 4 GiB).  A request larger than the room left in a step is split: the
+ * step takes its next rows and the rest stays at the head of the queue, so
+ * requests of any size are accepted and a large one runs on several steps at
+ * once.  Requests arriving while other steps run join the next step that has
+ * room, and steps of different workers overlap on the GPU.  max_rows bounds
+ * a step's duration and with it the latency a step adds to the requests
+ * queued behind it.  A request's results do not depend on the steps it
+ * joins.
  * random_mode: PPSK_RANDOM_NONE or PPSK_RANDOM_EQUAL_BASE (query lengths
  * passed at submit).  The reference set must outlive the server.
  */
@@ -261,8 +266,9 @@ typedef struct ppsk_result {
   float *thr_d;
   uint64_t n_fail;  /* pairs whose regression failed (distances 0, 0) */
   double t_submit, t_start, t_done; /* seconds since server creation */
-  int64_t step_id;   /* step that served the request */
+  int64_t step_id;   /* step that served the request's first rows */
   int64_t step_rows; /* query samples in that step */
+  int64_t n_steps;   /* steps that carried the request's rows */
   void *owner;       /* library-internal */
 } ppsk_result;
 

@@ -217,7 +217,7 @@ def serve(a):
     plain_rate = plain_pairs / plain_s
 
     srv = H.Server(ref, kNN=10, cutoff=0.05, random_mode=H.RANDOM_EQUAL_BASE,
-                   max_rows=1024, n_slots=4)
+                   max_rows=a.serve_max_rows, n_slots=a.serve_slots)
     try:
         # saturation: every request queued at once
         sat_req = draw(64)
@@ -278,7 +278,7 @@ def serve(a):
         "server": st,
         "config": {"workload": "serving, resident reference database", "n_refs": n_ref,
                    "kmers": KL8, "bins": S64 * 64, "request_sizes": "log-uniform 1..512",
-                   "kNN": 10, "threshold": 0.05, "slots": 4, "max_rows_per_step": 1024,
+                   "kNN": 10, "threshold": 0.05, "slots": a.serve_slots, "max_rows_per_step": a.serve_max_rows,
                    "random_correction": "equal-base"},
     }
 

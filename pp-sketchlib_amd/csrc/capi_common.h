@@ -115,6 +115,11 @@ void fill_common(ppsk::DistParams& p, const ppsk_set* rows, const ppsk_set* cols
                  const RandomIn& rnd, double tol);
 // 1 unless PPSK_ROW_PREFETCH=0 (row-panel L2 prefetch of the tile kernels).
 int row_prefetch();
+// Tiles of a persistent launch of n whole tiles over `grid` workgroups that
+// run instead as (tile, k-mer length) items of the per-(tile, k) kernel: the
+// last, partial round when it would fill at most half of the workgroup slots
+// (0 otherwise, or with PPSK_TAIL_SPLIT=0).
+size_t tail_split_tiles(size_t n, int grid);
 // Device copy of a Monte Carlo table (null for the other modes); caller frees.
 int upload_table(DeviceCtx* c, const ppsk_set* s, const RandomIn& rnd, float** dtab);
 

@@ -338,15 +338,12 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
   // of the slots, those tiles go to a second launch as (tile, k-mer length)
   // items of the per-(tile, k) kernel instead (PPSK_TAIL_SPLIT=0 turns this
   // off).  Same tile code, same epilogue, same output.
-  const char* split_env = std::getenv("PPSK_TAIL_SPLIT");
-  const bool split_tail = !split_env || std::atoi(split_env) != 0;
   std::vector<size_t> tail(chunks.size(), 0);
   std::vector<ppsk::Tile> tail_tiles;
   std::vector<size_t> tail_at(chunks.size(), 0);
   size_t most_tail = 0;
   for (size_t ci = 0; ci < chunks.size(); ++ci) {
-    const size_t n = chunks[ci].second - chunks[ci].first, r = n % (size_t)grid;
-    if (split_tail && n > (size_t)grid && r > 0 && r <= (size_t)grid / 2) tail[ci] = r;
+    tail[ci] = tail_split_tiles(chunks[ci].second - chunks[ci].first, grid);
     tail_at[ci] = tail_tiles.size();
     tail_tiles.insert(tail_tiles.end(), tiles.begin() + (chunks[ci].second - tail[ci]),
                       tiles.begin() + chunks[ci].second);
@@ -484,6 +481,13 @@ void fill_common(ppsk::DistParams& p, const ppsk_set* rows, const ppsk_set* cols
 int row_prefetch() {
   const char* e = std::getenv("PPSK_ROW_PREFETCH");
   return e ? (std::atoi(e) != 0) : 1;
+}
+
+size_t tail_split_tiles(size_t n, int grid) {
+  const char* e = std::getenv("PPSK_TAIL_SPLIT");
+  if (e && std::atoi(e) == 0) return 0;
+  const size_t g = (size_t)grid, r = g ? n % g : 0;
+  return (n > g && r > 0 && r <= g / 2) ? r : 0;
 }
 
 int upload_table(DeviceCtx* c, const ppsk_set* s, const RandomIn& rnd, float** dtab) {

@@ -34,10 +34,15 @@ hipError_t launch_knn_select_rect(const float* band, int64_t n_rows, int64_t n, 
 hipError_t launch_threshold_count(const float2* dd, int64_t rows, int64_t n, bool rect,
                                   int dist_col, float cutoff, int64_t* counts,
                                   hipStream_t stream);
+// Entries at offsets >= capacity are not written (the caller sizes the
+// outputs from the total and writes again).
 hipError_t launch_threshold_write(const float2* dd, int64_t rows, int64_t n, bool rect,
                                   int dist_col, float cutoff, const int64_t* offsets,
                                   int64_t* i_vec, int64_t* j_vec, float* dists,
-                                  hipStream_t stream);
+                                  hipStream_t stream, int64_t capacity = INT64_MAX);
+// Per-row counts -> exclusive offsets in place, total in counts[rows]
+// (device memory of rows + 1 entries; one workgroup).
+hipError_t launch_offsets_scan(int64_t* counts, int64_t rows, hipStream_t stream);
 
 // dd: condensed float2 (core, acc) of the self upper triangle of n samples.
 // Allocates the device outputs (*d_i, *d_j, *d_d; caller frees with hipFree).

@@ -218,12 +218,46 @@ __global__ __launch_bounds__(TB) void thr_count_kernel(const float2* __restrict_
   }
 }
 
+// Exclusive scan of counts[0 .. rows) in place and the total in
+// counts[rows]: one workgroup walks the rows in chunks of TB (a serving step
+// has a few hundred rows).
+__global__ __launch_bounds__(TB) void offsets_scan_kernel(int64_t* __restrict__ counts,
+                                                          int64_t rows) {
+  __shared__ int64_t wave_sum[TB / 64];
+  __shared__ int64_t carry_sh;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry_sh = 0;
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < rows; c0 += TB) {
+    const int64_t t = c0 + threadIdx.x;
+    const int64_t v = t < rows ? counts[t] : 0;
+    int64_t x = v;  // inclusive scan inside the wave
+    for (int off = 1; off < 64; off <<= 1) {
+      const int64_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wave_sum[wave] = x;
+    __syncthreads();
+    int64_t before = carry_sh, chunk = 0;
+    for (int w = 0; w < TB / 64; ++w) {
+      if (w < wave) before += wave_sum[w];
+      chunk += wave_sum[w];
+    }
+    if (t < rows) counts[t] = before + x - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry_sh += chunk;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[rows] = carry_sh;
+}
+
 __global__ __launch_bounds__(TB) void thr_write_kernel(const float2* __restrict__ dd, int64_t n,
                                                        bool rect, int col, float cutoff,
                                                        const int64_t* __restrict__ offsets,
                                                        int64_t* __restrict__ iv,
                                                        int64_t* __restrict__ jv,
-                                                       float* __restrict__ dv) {
+                                                       float* __restrict__ dv, int64_t cap) {
   const int64_t i = blockIdx.x;
   const RowSpan sp = row_span(rect, i, n);
   __shared__ int wave_tot[TB / 64];
@@ -251,8 +285,8 @@ __global__ __launch_bounds__(TB) void thr_write_kernel(const float2* __restrict_
       chunk_tot += wave_tot[w];
     }
     const int64_t base = base_sh;
-    if (keep) {
-      const int64_t o = base + wave_off + before;
+    const int64_t o = base + wave_off + before;
+    if (keep && o < cap) {
       iv[o] = i;
       jv[o] = sp.j0 + t;
       dv[o] = d;
@@ -274,10 +308,16 @@ hipError_t launch_threshold_count(const float2* dd, int64_t rows, int64_t n, boo
 hipError_t launch_threshold_write(const float2* dd, int64_t rows, int64_t n, bool rect,
                                   int dist_col, float cutoff, const int64_t* offsets,
                                   int64_t* i_vec, int64_t* j_vec, float* dists,
-                                  hipStream_t stream) {
+                                  hipStream_t stream, int64_t capacity) {
   if (rows <= 0) return hipSuccess;
   thr_write_kernel<<<(unsigned)rows, TB, 0, stream>>>(dd, n, rect, dist_col, cutoff, offsets,
-                                                       i_vec, j_vec, dists);
+                                                       i_vec, j_vec, dists, capacity);
+  return hipGetLastError();
+}
+
+hipError_t launch_offsets_scan(int64_t* counts, int64_t rows, hipStream_t stream) {
+  if (rows < 0) return hipErrorInvalidValue;
+  offsets_scan_kernel<<<1, TB, 0, stream>>>(counts, rows);
   return hipGetLastError();
 }
 

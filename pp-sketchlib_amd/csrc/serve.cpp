@@ -2,20 +2,30 @@
 // HBM and query batches arrive at any time.  Continuous batching:
 //
 //  * every request waits in one FIFO queue;
-//  * n_slots worker threads, each with its own stream, repeatedly take as
-//    many queued requests as fit one step (at most max_rows query samples
-//    and the step's device memory, carved from one arena allocated at
-//    start-up), so requests that arrive while other steps are on the GPU are
-//    admitted as soon as a slot and memory free up, and steps of different
-//    slots overlap on the device;
-//  * the queries of all requests in a step share the row tiles of one
-//    query-vs-reference distance pass (the dense band never leaves the
-//    device), followed by kNN selection and threshold compaction straight
-//    into page-locked host buffers;
-//  * each request's rows are a contiguous slice of the step, so its results
-//    are the slice of the step's results, re-indexed to the request.
+//  * n_slots worker threads, each with its own stream, repeatedly fill one
+//    step with query rows, oldest request first: at most max_rows rows and
+//    the step's device memory, carved from one arena allocated at start-up.
+//    A request that does not fit the step's remaining rows contributes a
+//    piece (its next rows) and stays at the head of the queue, so a large
+//    request spreads over several steps, which other slots run at the same
+//    time, and no step grows beyond max_rows;
+//  * requests that arrive while other steps are on the GPU are admitted as
+//    soon as a slot and memory free up, and steps of different slots overlap
+//    on the device;
+//  * the query rows of a step share the row tiles of one query-vs-reference
+//    distance pass (the dense band never leaves the device), followed by kNN
+//    selection and threshold compaction straight into page-locked host
+//    buffers;
+//  * each piece is a contiguous slice of the step's rows, so its results are
+//    that slice of the step's results, re-indexed to the request; a request
+//    is complete when the step of its last piece has finished.
 //
-// Results do not depend on the step a request lands in: every output value
+// The step budget bounds latency: the persistent distance kernel holds every
+// CU for its whole pass, so work queued behind a step (another step's query
+// striping, kNN, threshold kernels) waits for it.  Small steps keep that wait
+// short; a few dozen query rows already keep the pass compute-bound.
+//
+// Results do not depend on the steps a request lands in: every output value
 // is a function of its own (query, reference) pair only.
 #include <hip/hip_runtime.h>
 
@@ -97,6 +107,12 @@ double now_s(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Threshold hits of one piece, query index relative to the request.
+struct ThrPart {
+  std::vector<int64_t> i, j;
+  std::vector<float> d;
+};
+
 }  // namespace
 
 struct ppsk_request {
@@ -105,8 +121,12 @@ struct ppsk_request {
   std::vector<uint64_t> raw;  // [n_q][n_k][s64 * 14]
   std::vector<double> jv;     // [n_q][n_k] equal-base terms
   ppsk_result res;            // public view of the vectors below
+  // kNN lists are sized at submit; each piece fills its own rows in place
   std::vector<int64_t> ki, kj, ti, tj;
   std::vector<float> kd, td;
+  int64_t next_row = 0;   // first row not yet given to a step
+  int64_t rows_done = 0;  // rows whose step has finished
+  std::map<int64_t, ThrPart> thr_parts;  // by the piece's first row
   bool done = false;
   int status = PPSK_OK;
   std::string error;
@@ -154,12 +174,15 @@ int step_variant(const ppsk_server* s, int64_t rows) {
 
 // Device memory of one step, carved from the arena.  A step with at least
 // two tiles per resident workgroup runs the persistent tile kernel (whole
-// tiles from work queues, per-workgroup count blocks); a smaller one is split
+// tiles from work queues, per-workgroup count blocks), its last partial round
+// as (tile, k-mer length) items when that round would leave most workgroup
+// slots idle (tail tiles, per-tile count scratch); a smaller step is split
 // per (tile, k-mer length) over more workgroups (per-tile count scratch,
 // launched in chunks of chunk_tiles).
 struct StepLayout {
-  size_t staging, stripes, jv, band, tiles, scratch, tile_done, fail, fail_rows, params, total;
-  int64_t nbq, n_tiles, chunk_tiles;
+  size_t staging, stripes, jv, band, tiles, scratch, tile_done, fail, fail_rows, params;
+  size_t tail_tiles, tail_counts, tail_done, counts, total;
+  int64_t nbq, n_tiles, chunk_tiles, tail;
   bool persistent;
   int grid;
 };
@@ -187,10 +210,16 @@ StepLayout layout(const ppsk_server* s, int64_t rows) {
   L.jv = take((size_t)rows * r->n_k * sizeof(double));
   L.band = take((size_t)rows * r->n * 2 * sizeof(float));
   if (L.persistent) {
-    L.tiles = take((size_t)L.n_tiles * sizeof(ppsk::BatchItem));
+    L.tail = (int64_t)tail_split_tiles((size_t)L.n_tiles, L.grid);
+    L.tiles = take((size_t)(L.n_tiles - L.tail) * sizeof(ppsk::BatchItem));
     L.scratch = take((size_t)L.grid * per_tile);
     L.tile_done = take(ppsk::BATCH_QUEUES * sizeof(uint32_t));  // queue counters
     L.params = take(sizeof(ppsk::DistParams));
+    if (L.tail > 0) {
+      L.tail_tiles = take((size_t)L.tail * sizeof(ppsk::Tile));
+      L.tail_counts = take((size_t)L.tail * per_tile);
+      L.tail_done = take((size_t)L.tail * sizeof(uint32_t));
+    }
   } else {
     L.tiles = take((size_t)L.n_tiles * sizeof(ppsk::Tile));
     L.scratch = take((size_t)L.chunk_tiles * per_tile);
@@ -199,6 +228,7 @@ StepLayout layout(const ppsk_server* s, int64_t rows) {
   }
   L.fail = take(2 * sizeof(unsigned long long));
   L.fail_rows = take((size_t)rows * sizeof(uint32_t));
+  L.counts = take((size_t)(rows + 1) * sizeof(int64_t));
   L.total = off;
   return L;
 }
@@ -230,6 +260,15 @@ struct Worker {
   Pinned knn_i, knn_j, knn_d, counts, thr_i, thr_j, thr_d, fail_rows;
 };
 
+// Rows [q0, q0 + n) of a request, carried by one step; thr and nf are the
+// piece's results until they are merged into the request.
+struct Piece {
+  ppsk_request* r;
+  int64_t q0, n;
+  ThrPart thr;
+  uint64_t nf = 0;
+};
+
 #define STEP_TRY(expr)                                                 \
   do {                                                                 \
     hipError_t _e = (expr);                                            \
@@ -239,9 +278,9 @@ struct Worker {
     }                                                                  \
   } while (0)
 
-// One step over `batch` (rows contiguous in submission order).
-int run_step(ppsk_server* s, Worker& w, const std::vector<ppsk_request*>& batch, int64_t rows,
-             char* mem, const StepLayout& L, std::string& err) {
+// One step over `batch` (the pieces' rows back to back in batch order).
+int run_step(ppsk_server* s, Worker& w, std::vector<Piece>& batch, int64_t rows, char* mem,
+             const StepLayout& L, std::string& err) {
   const ppsk_set* ref = s->ref;
   const int n_k = ref->n_k;
   uint64_t* staging = (uint64_t*)(mem + L.staging);
@@ -255,15 +294,17 @@ int run_step(ppsk_server* s, Worker& w, const std::vector<ppsk_request*>& batch,
   uint32_t* fail_rows = (uint32_t*)(mem + L.fail_rows);
 
   // queries -> stripes (+ equal-base terms)
+  const size_t words = s->per_sample / sizeof(uint64_t);
   int64_t o = 0;
-  for (ppsk_request* r : batch) {
-    STEP_TRY(hipMemcpyAsync(staging + (size_t)o * (s->per_sample / sizeof(uint64_t)), r->raw.data(),
-                            (size_t)r->n_q * s->per_sample, hipMemcpyHostToDevice, w.st));
+  for (const Piece& pc : batch) {
+    const ppsk_request* r = pc.r;
+    STEP_TRY(hipMemcpyAsync(staging + (size_t)o * words, r->raw.data() + (size_t)pc.q0 * words,
+                            (size_t)pc.n * s->per_sample, hipMemcpyHostToDevice, w.st));
     if (!r->jv.empty()) {
-      STEP_TRY(hipMemcpyAsync(jv + (size_t)o * n_k, r->jv.data(), r->jv.size() * sizeof(double),
-                              hipMemcpyHostToDevice, w.st));
+      STEP_TRY(hipMemcpyAsync(jv + (size_t)o * n_k, r->jv.data() + (size_t)pc.q0 * n_k,
+                              (size_t)pc.n * n_k * sizeof(double), hipMemcpyHostToDevice, w.st));
     }
-    o += r->n_q;
+    o += pc.n;
   }
   STEP_TRY(ppsk::launch_stripe(staging, rows, 0, n_k, ref->s64, (int32_t)L.nbq, stripes, w.st));
   STEP_TRY(hipMemsetAsync(tile_done, 0, (size_t)L.chunk_tiles * sizeof(uint32_t), w.st));
@@ -303,11 +344,12 @@ int run_step(ppsk_server* s, Worker& w, const std::vector<ppsk_request*>& batch,
   const auto tiles = make_tiles(rows, ref->n, tr, tc, false, 0, rows, tile_group(), &gs);
   if (L.persistent) {
     // whole tiles from 8 queues, each a contiguous eighth of the tile list
+    // less its tail
     std::vector<ppsk::BatchItem> items;
     items.reserve(tiles.size());
     ppsk::BatchArgs args;
     std::memset(&args, 0, sizeof(args));
-    const size_t n = tiles.size();
+    const size_t n = tiles.size() - (size_t)L.tail;
     for (int q = 0; q < ppsk::BATCH_QUEUES; ++q) {
       args.queue_begin[q] = (int64_t)items.size();
       for (size_t x = n * q / ppsk::BATCH_QUEUES; x < n * (q + 1) / ppsk::BATCH_QUEUES; ++x) {
@@ -324,9 +366,25 @@ int run_step(ppsk_server* s, Worker& w, const std::vector<ppsk_request*>& batch,
     STEP_TRY(hipMemcpyAsync(mem + L.tiles, items.data(), items.size() * sizeof(ppsk::BatchItem),
                             hipMemcpyHostToDevice, w.st));
     STEP_TRY(hipMemcpyAsync(d_params, &p, sizeof(p), hipMemcpyHostToDevice, w.st));
+    if (L.tail > 0) {
+      STEP_TRY(hipMemcpyAsync(mem + L.tail_tiles, tiles.data() + n, (size_t)L.tail * sizeof(ppsk::Tile),
+                              hipMemcpyHostToDevice, w.st));
+      STEP_TRY(hipMemsetAsync(mem + L.tail_done, 0, (size_t)L.tail * sizeof(uint32_t), w.st));
+    }
     STEP_TRY(ppsk::launch_batch_reset(tile_done, nullptr, 0, w.st));
     STEP_TRY(hipEventRecord(w.e0, w.st));
     STEP_TRY(ppsk::launch_batch(args, variant, L.grid, w.st));
+    if (L.tail > 0) {
+      // the last partial round as (tile, k) items, after the persistent
+      // launch on the same stream
+      ppsk::DistParams pt = p;
+      pt.tiles = (const ppsk::Tile*)(mem + L.tail_tiles);
+      pt.n_tiles = L.tail;
+      pt.item_base = 0;
+      pt.counts = (uint16_t*)(mem + L.tail_counts);
+      pt.tile_done = (uint32_t*)(mem + L.tail_done);
+      STEP_TRY(ppsk::launch_dists(pt, variant, L.tail * n_k, w.st));
+    }
     STEP_TRY(hipEventRecord(w.e1, w.st));
   } else {
     STEP_TRY(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(ppsk::Tile),
@@ -357,103 +415,143 @@ int run_step(ppsk_server* s, Worker& w, const std::vector<ppsk_request*>& batch,
     err = "page-locked count buffers";
     return PPSK_ERR_MEMORY;
   }
+  // threshold: per-row counts, offsets scanned on the device, hits written
+  // straight to page-locked buffers of the capacity they already have, so the
+  // whole step is queued at once and waited for once; only a step with more
+  // hits than that capacity writes again into larger buffers
   int64_t* cnt = (int64_t*)w.counts.p;
+  int64_t* dcnt = (int64_t*)(mem + L.counts);
+  int64_t cap = 0;
   if (s->cutoff > 0) {
+    const size_t want = std::max<size_t>((size_t)rows * 64, (size_t)1 << 16);
+    if (w.thr_j.n / 8 < want &&
+        (!w.thr_i.reserve(want * 8) || !w.thr_j.reserve(want * 8) || !w.thr_d.reserve(want * 4))) {
+      err = "page-locked threshold buffers";
+      return PPSK_ERR_MEMORY;
+    }
+    cap = (int64_t)std::min({w.thr_i.n / 8, w.thr_j.n / 8, w.thr_d.n / 4});
     STEP_TRY(ppsk::launch_threshold_count((const float2*)band, rows, ref->n, true, s->thr_col,
-                                          s->cutoff, cnt, w.st));
+                                          s->cutoff, dcnt, w.st));
+    STEP_TRY(ppsk::launch_offsets_scan(dcnt, rows, w.st));
+    STEP_TRY(ppsk::launch_threshold_write((const float2*)band, rows, ref->n, true, s->thr_col,
+                                          s->cutoff, dcnt, (int64_t*)w.thr_i.p,
+                                          (int64_t*)w.thr_j.p, (float*)w.thr_d.p, w.st, cap));
+    STEP_TRY(hipMemcpyAsync(cnt, dcnt, (size_t)(rows + 1) * 8, hipMemcpyDeviceToHost, w.st));
   }
   STEP_TRY(hipMemcpyAsync(w.fail_rows.p, fail_rows, (size_t)rows * 4, hipMemcpyDeviceToHost, w.st));
   STEP_TRY(hipStreamSynchronize(w.st));
   float ms = 0.f;
   STEP_TRY(hipEventElapsedTime(&ms, w.e0, w.e1));
-  int64_t total = 0;
-  if (s->cutoff > 0) {
-    for (int64_t x = 0; x < rows; ++x) {
-      const int64_t c = cnt[x];
-      cnt[x] = total;
-      total += c;
+  const int64_t total = s->cutoff > 0 ? cnt[rows] : 0;
+  if (total > cap) {
+    if (!w.thr_i.reserve((size_t)total * 8) || !w.thr_j.reserve((size_t)total * 8) ||
+        !w.thr_d.reserve((size_t)total * 4)) {
+      err = "page-locked threshold buffers";
+      return PPSK_ERR_MEMORY;
     }
-    cnt[rows] = total;
-    if (total > 0) {
-      if (!w.thr_i.reserve((size_t)total * 8) || !w.thr_j.reserve((size_t)total * 8) ||
-          !w.thr_d.reserve((size_t)total * 4)) {
-        err = "page-locked threshold buffers";
-        return PPSK_ERR_MEMORY;
-      }
-      STEP_TRY(ppsk::launch_threshold_write((const float2*)band, rows, ref->n, true, s->thr_col,
-                                            s->cutoff, cnt, (int64_t*)w.thr_i.p,
-                                            (int64_t*)w.thr_j.p, (float*)w.thr_d.p, w.st));
-      STEP_TRY(hipStreamSynchronize(w.st));
-    }
+    STEP_TRY(ppsk::launch_threshold_write((const float2*)band, rows, ref->n, true, s->thr_col,
+                                          s->cutoff, dcnt, (int64_t*)w.thr_i.p,
+                                          (int64_t*)w.thr_j.p, (float*)w.thr_d.p, w.st));
+    STEP_TRY(hipStreamSynchronize(w.st));
   }
 
-  // each request's slice, re-indexed to its own query numbering
+  // each piece's slice of the step, re-indexed to its request's query
+  // numbering; kNN rows go straight to the request's lists (pieces of one
+  // request never overlap), threshold hits and failures stay with the piece
   const uint32_t* fr = (const uint32_t*)w.fail_rows.p;
   o = 0;
-  for (ppsk_request* r : batch) {
-    const int64_t q0 = o, q1 = o + r->n_q;
+  for (Piece& pc : batch) {
+    ppsk_request* r = pc.r;
+    const int64_t s0 = o, s1 = o + pc.n, shift = pc.q0 - s0;
     if (s->kNN > 0) {
       const int64_t* si = (const int64_t*)w.knn_i.p;
       const int64_t* sj = (const int64_t*)w.knn_j.p;
       const float* sd = (const float*)w.knn_d.p;
-      const size_t a = (size_t)q0 * s->kNN, b = (size_t)q1 * s->kNN;
-      r->ki.resize(b - a);
-      for (size_t x = a; x < b; ++x) r->ki[x - a] = si[x] - q0;
-      r->kj.assign(sj + a, sj + b);
-      r->kd.assign(sd + a, sd + b);
+      const size_t a = (size_t)s0 * s->kNN, b = (size_t)s1 * s->kNN;
+      const size_t at = (size_t)pc.q0 * s->kNN;
+      for (size_t x = a; x < b; ++x) r->ki[at + x - a] = si[x] + shift;
+      std::copy(sj + a, sj + b, r->kj.begin() + at);
+      std::copy(sd + a, sd + b, r->kd.begin() + at);
     }
     if (s->cutoff > 0 && total > 0) {
-      const int64_t a = cnt[q0], b = cnt[q1];
+      const int64_t a = cnt[s0], b = cnt[s1];
       const int64_t* si = (const int64_t*)w.thr_i.p;
-      r->ti.resize(b - a);
-      for (int64_t x = a; x < b; ++x) r->ti[x - a] = si[x] - q0;
-      r->tj.assign((const int64_t*)w.thr_j.p + a, (const int64_t*)w.thr_j.p + b);
-      r->td.assign((const float*)w.thr_d.p + a, (const float*)w.thr_d.p + b);
+      pc.thr.i.resize(b - a);
+      for (int64_t x = a; x < b; ++x) pc.thr.i[x - a] = si[x] + shift;
+      pc.thr.j.assign((const int64_t*)w.thr_j.p + a, (const int64_t*)w.thr_j.p + b);
+      pc.thr.d.assign((const float*)w.thr_d.p + a, (const float*)w.thr_d.p + b);
     }
     uint64_t nf = 0;
-    for (int64_t x = q0; x < q1; ++x) nf += fr[x];
-    r->res.n_fail = nf;
-    o = q1;
+    for (int64_t x = s0; x < s1; ++x) nf += fr[x];
+    pc.nf = nf;
+    o = s1;
   }
   std::lock_guard<std::mutex> lk(s->mu);
   s->gpu_ms += ms;
   return PPSK_OK;
 }
 
+// Rows of the next piece of the oldest request in a step of its own.
+int64_t head_piece_rows(const ppsk_server* s) {
+  const ppsk_request* r = s->pending.front();
+  return std::min<int64_t>(r->n_q - r->next_row, s->max_rows);
+}
+
+// All pieces of r have finished: threshold hits in row order, done.
+void finish_request(ppsk_request* r, double t) {
+  size_t n = 0;
+  for (auto& kv : r->thr_parts) n += kv.second.j.size();
+  r->ti.reserve(n);
+  r->tj.reserve(n);
+  r->td.reserve(n);
+  for (auto& kv : r->thr_parts) {
+    r->ti.insert(r->ti.end(), kv.second.i.begin(), kv.second.i.end());
+    r->tj.insert(r->tj.end(), kv.second.j.begin(), kv.second.j.end());
+    r->td.insert(r->td.end(), kv.second.d.begin(), kv.second.d.end());
+  }
+  r->thr_parts.clear();
+  r->res.t_done = t;
+  r->done = true;
+}
+
 void worker_main(ppsk_server* s, Worker* w) {
   hipSetDevice(s->device);
   for (;;) {
-    std::vector<ppsk_request*> batch;
+    std::vector<Piece> batch;
     int64_t rows = 0;
     size_t off = 0;
     StepLayout L{};
     {
       std::unique_lock<std::mutex> lk(s->mu);
       // admission: a free slot (this worker) and device memory for at least
-      // the oldest request
+      // the next piece of the oldest request
       s->cv_work.wait(lk, [&] {
         return s->stop ||
-               (!s->pending.empty() && s->arena.fits(layout(s, s->pending.front()->n_q).total));
+               (!s->pending.empty() && s->arena.fits(layout(s, head_piece_rows(s)).total));
       });
       if (s->stop) return;
-      // continuous batching: oldest first, while rows and memory allow
-      while (!s->pending.empty()) {
+      // continuous batching: oldest first, while rows and memory allow; a
+      // request larger than the room left gives a piece and keeps its place
+      const int64_t step_id = s->steps;
+      while (!s->pending.empty() && rows < s->max_rows) {
         ppsk_request* r = s->pending.front();
-        const int64_t nr = rows + r->n_q;
-        if (nr > s->max_rows && rows > 0) break;
-        if (!s->arena.fits(layout(s, nr).total)) break;
-        batch.push_back(r);
-        rows = nr;
-        s->pending.pop_front();
+        const int64_t take = std::min<int64_t>(r->n_q - r->next_row, s->max_rows - rows);
+        if (!s->arena.fits(layout(s, rows + take).total)) break;
+        if (r->next_row == 0) {
+          r->res.t_start = now_s(s->t0);
+          r->res.step_id = step_id;
+        }
+        batch.push_back({r, r->next_row, take});
+        r->next_row += take;
+        r->res.n_steps += 1;
+        rows += take;
+        if (r->next_row == r->n_q) s->pending.pop_front();
+      }
+      for (const Piece& pc : batch) {
+        if (pc.q0 == 0) pc.r->res.step_rows = rows;
       }
       L = layout(s, rows);
       s->arena.alloc(L.total, &off);  // fits() held under the same lock
-      const double t = now_s(s->t0);
-      for (ppsk_request* r : batch) {
-        r->res.t_start = t;
-        r->res.step_rows = rows;
-        r->res.step_id = s->steps;
-      }
       s->steps += 1;
       s->inflight += 1;
       s->peak_inflight = std::max(s->peak_inflight, s->inflight);
@@ -465,14 +563,24 @@ void worker_main(ppsk_server* s, Worker* w) {
       s->arena.release(off, L.total);
       s->inflight -= 1;
       const double t = now_s(s->t0);
-      for (ppsk_request* r : batch) {
-        r->status = rc;
-        r->error = err;
-        r->res.t_done = t;
-        r->done = true;
+      for (Piece& pc : batch) {
+        ppsk_request* r = pc.r;
+        if (rc != PPSK_OK) {
+          if (r->status == PPSK_OK) {
+            r->status = rc;
+            r->error = err;
+          }
+        } else {
+          r->res.n_fail += pc.nf;
+          if (!pc.thr.j.empty()) r->thr_parts[pc.q0] = std::move(pc.thr);
+        }
+        r->rows_done += pc.n;
+        if (r->rows_done == r->n_q) {
+          finish_request(r, t);
+          s->requests_done += 1;
+        }
       }
       s->rows_done += rows;
-      s->requests_done += (int64_t)batch.size();
     }
     s->cv_done.notify_all();
     s->cv_work.notify_all();
@@ -567,10 +675,9 @@ int ppsk_server_submit(ppsk_server* s, int64_t n_queries, const uint64_t* raw,
                        const uint64_t* lengths, int64_t* ticket) {
   if (!s || !ticket) return fail(PPSK_ERR_ARG, "server / ticket is NULL");
   if (n_queries < 1 || !raw) return fail(PPSK_ERR_ARG, "a request needs at least one query sketch");
-  if (n_queries > s->max_rows) {
-    return fail(PPSK_ERR_ARG, "request larger than max_rows (" + std::to_string(s->max_rows) + ")");
-  }
-  if (layout(s, n_queries).total > s->arena.size) {
+  // a request larger than max_rows is served in pieces of at most max_rows
+  // rows, so only a piece has to fit the arena
+  if (layout(s, std::min<int64_t>(n_queries, s->max_rows)).total > s->arena.size) {
     return fail(PPSK_ERR_MEMORY, "request needs more device memory than the serving arena");
   }
   const ppsk_set* ref = s->ref;
@@ -591,6 +698,12 @@ int ppsk_server_submit(ppsk_server* s, int64_t n_queries, const uint64_t* raw,
         r->jv[(size_t)q * ref->n_k + kk] = 1 - std::pow(base, (double)lengths[q]);
       }
     }
+  }
+  if (s->kNN > 0) {
+    const size_t nk = (size_t)n_queries * s->kNN;
+    r->ki.resize(nk);
+    r->kj.resize(nk);
+    r->kd.resize(nk);
   }
   std::memset(&r->res, 0, sizeof(r->res));
   {

@@ -50,7 +50,7 @@ class Result(_c.Structure):
                 ("thr_n", _i64), ("thr_i", _c.POINTER(_i64)), ("thr_j", _c.POINTER(_i64)),
                 ("thr_d", _c.POINTER(_c.c_float)), ("n_fail", _u64), ("t_submit", _c.c_double),
                 ("t_start", _c.c_double), ("t_done", _c.c_double), ("step_id", _i64),
-                ("step_rows", _i64), ("owner", _p)]
+                ("step_rows", _i64), ("n_steps", _i64), ("owner", _p)]
 
 
 class PpskError(RuntimeError):
@@ -351,7 +351,7 @@ class Server:
     the request's timestamps."""
 
     def __init__(self, ref, kNN=10, knn_col=0, cutoff=0.05, thr_col=0,
-                 random_mode=RANDOM_NONE, tol=None, max_rows=1024, n_slots=4, arena_bytes=0):
+                 random_mode=RANDOM_NONE, tol=None, max_rows=128, n_slots=2, arena_bytes=0):
         self.ref = ref
         self.random_mode = random_mode
         tol = default_tolerance(ref.s64) if tol is None else tol
@@ -381,7 +381,8 @@ class Server:
                 thr_j=_copy_out(x.thr_j, x.thr_n, np.int64),
                 thr_d=_copy_out(x.thr_d, x.thr_n, np.float32),
                 n_fail=x.n_fail, t_submit=x.t_submit, t_start=x.t_start, t_done=x.t_done,
-                step_id=x.step_id, step_rows=x.step_rows, n_queries=x.n_queries)
+                step_id=x.step_id, step_rows=x.step_rows, n_steps=x.n_steps,
+                n_queries=x.n_queries)
         finally:
             lib().ppsk_result_free(r)
         return out

@@ -40,7 +40,7 @@ def _plain(ref, q_raw, q_le):
     with H.SketchSet(q_raw, KLIST8, S64, lengths=q_le) as q:
         d, nf, _ = H.dense(ref, q, random_mode=H.RANDOM_EQUAL_BASE)
     nq = q_raw.shape[0]
-    core = d[:, 0].reshape(nq, N_REF)
+    core = d[:, 0].reshape(nq, ref.n)
     ki, kj, kd = O.knn_rows(core, np.full(nq, -1), KNN)
     ti, tj = np.nonzero(core < CUTOFF)
     td = core[ti, tj]
@@ -129,12 +129,42 @@ def test_server_against_oracle(data):
     assert set(zip(ti[sel].tolist(), tj[sel].tolist())) <= got
 
 
+@pytest.mark.parametrize("n_slots", [1, 3])
+def test_requests_split_over_steps(data, n_slots):
+    # requests larger than the room in a step are served in pieces of at most
+    # max_rows rows, pieces of different requests sharing steps; each result
+    # equals the same request served in one step of its own
+    ref, q_raw, q_le = data["ref"], data["q_raw"], data["q_le"]
+    spans = [(10, 60), (200, 207), (300, 333), (400, 420), (500, 501), (600, 664)]
+    with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
+                  max_rows=64, n_slots=1) as whole:
+        want = [whole.wait(whole.submit(q_raw[s:e], q_le[s:e])) for s, e in spans]
+    assert all(w["n_steps"] == 1 for w in want)
+    with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
+                  max_rows=24, n_slots=n_slots) as srv:
+        tickets = [srv.submit(q_raw[s:e], q_le[s:e]) for s, e in spans]
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    for (s, e), w, r in zip(spans, want, res):
+        _same(w, r)
+        assert r["n_queries"] == e - s
+        assert r["n_steps"] >= -(-(e - s) // 24)
+    assert st["rows"] == sum(e - s for s, e in spans)
+    assert st["requests"] == len(spans)
+    assert st["steps"] >= -(-st["rows"] // 24)  # no step above max_rows
+    # the 64-query request against the plain path
+    _, _, (ki, kj, kd), (ti, tj, td) = _plain(ref, q_raw[600:664], q_le[600:664])
+    r = res[-1]
+    assert np.array_equal(r["knn_i"], ki) and np.array_equal(r["knn_j"], kj)
+    assert np.array_equal(r["knn_d"].view(np.uint32), kd.view(np.uint32))
+    assert np.array_equal(r["thr_i"], ti) and np.array_equal(r["thr_j"], tj)
+    assert np.array_equal(r["thr_d"].view(np.uint32), td.view(np.uint32))
+
+
 def test_server_rejects_bad_requests(data):
     ref, q_raw, q_le = data["ref"], data["q_raw"], data["q_le"]
     with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
                   max_rows=16, n_slots=1) as srv:
-        with pytest.raises(H.PpskError, match="max_rows"):
-            srv.submit(q_raw[:17], q_le[:17])
         with pytest.raises(H.PpskError, match="lengths"):
             srv.submit(q_raw[:2])
         with pytest.raises(H.PpskError, match="ticket"):
@@ -162,3 +192,51 @@ def test_admission_by_memory(data):
         _same(a, b)
     with pytest.raises(H.PpskError, match="arena"):
         H.Server(ref, kNN=KNN, cutoff=CUTOFF, max_rows=64, n_slots=1, arena_bytes=1 << 20)
+
+
+def test_persistent_step_tail_split(monkeypatch):
+    # 33,000 references: a 512-query step is 8 x 129 tiles of 64 x 256 on the
+    # persistent kernel, two full rounds of its 512 resident workgroups and 8
+    # tiles more, which run as (tile, k-mer length) items after it; results
+    # equal the step without that split and the plain path
+    n_ref, n_q = 33_000, 512
+    # core divergence up to 15% keeps the 0.05 threshold to close relatives
+    raw, le, _ = H.synth(n_ref + n_q, KLIST8, S64, seed=23, dcore_max=0.15)
+    q_raw, q_le = raw[n_ref:].copy(), le[n_ref:].copy()
+    with H.SketchSet(raw[:n_ref], KLIST8, S64, lengths=le[:n_ref]) as ref:
+        del raw
+        res = {}
+        for split in ("1", "0"):
+            monkeypatch.setenv("PPSK_TAIL_SPLIT", split)
+            with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
+                          max_rows=n_q, n_slots=1) as srv:
+                res[split] = srv.wait(srv.submit(q_raw, q_le))
+            assert res[split]["n_steps"] == 1
+        _same(res["1"], res["0"])
+        _, nf, (ki, kj, kd), (ti, tj, td) = _plain(ref, q_raw, q_le)
+    r = res["1"]
+    assert r["n_fail"] == nf
+    assert np.array_equal(r["knn_i"], ki) and np.array_equal(r["knn_j"], kj)
+    assert np.array_equal(r["knn_d"].view(np.uint32), kd.view(np.uint32))
+    assert np.array_equal(r["thr_i"], ti) and np.array_equal(r["thr_j"], tj)
+    assert np.array_equal(r["thr_d"].view(np.uint32), td.view(np.uint32))
+
+
+def test_threshold_hits_beyond_buffer_capacity(data):
+    # cutoff 1.0 keeps nearly every pair: a 64-query step has ~192,000 hits,
+    # more than the page-locked buffers first hold (64 per row, at least
+    # 65,536), so the step writes its hits again into larger buffers; the next
+    # step fits them at once
+    ref, q_raw, q_le = data["ref"], data["q_raw"], data["q_le"]
+    spans = [(0, 64), (64, 128)]
+    with H.Server(ref, kNN=KNN, cutoff=1.0, random_mode=H.RANDOM_EQUAL_BASE,
+                  max_rows=64, n_slots=1) as srv:
+        res = [srv.wait(srv.submit(q_raw[s:e], q_le[s:e])) for s, e in spans]
+    for (s, e), r in zip(spans, res):
+        with H.SketchSet(q_raw[s:e], KLIST8, S64, lengths=q_le[s:e]) as q:
+            d, _, _ = H.dense(ref, q, random_mode=H.RANDOM_EQUAL_BASE)
+        core = d[:, 0].reshape(e - s, N_REF)
+        ti, tj = np.nonzero(core < 1.0)
+        assert len(ti) > 65536
+        assert np.array_equal(r["thr_i"], ti) and np.array_equal(r["thr_j"], tj)
+        assert np.array_equal(r["thr_d"].view(np.uint32), core[ti, tj].view(np.uint32))
