@@ -514,6 +514,87 @@ void finish_request(ppsk_request* r, double t) {
   r->done = true;
 }
 
+// One wave that sleeps `n` times 127 x 64 clock cycles (about 2 ms for
+// n = 600 at 2.4 GHz).
+__global__ void queue_probe_kernel(int n) {
+  for (int x = 0; x < n; ++x) __builtin_amdgcn_s_sleep(127);
+}
+
+// Whether work on streams a and b runs at the same time: a wait on each, both
+// queued at once, ends about one wait after the first starts when the streams
+// feed different hardware queues and about two waits after it when they share
+// one (the runtime maps streams onto a few queues, so two streams of one
+// process may be served one after the other).
+int streams_concurrent(hipStream_t a, hipStream_t b, int ticks, bool* concurrent) {
+  hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+  int rc = PPSK_OK;
+  for (auto& x : e) {
+    if (hipEventCreate(&x) != hipSuccess) rc = PPSK_ERR_HIP;
+  }
+  float ta = 0.f, tb = 0.f;
+  if (rc == PPSK_OK) {
+    queue_probe_kernel<<<1, 64, 0, a>>>(0);  // first launch on the stream
+    if (hipStreamSynchronize(a) != hipSuccess || hipEventRecord(e[0], a) != hipSuccess) {
+      rc = PPSK_ERR_HIP;
+    }
+  }
+  if (rc == PPSK_OK) {
+    // a's end event before b's wait: on a shared queue, anything queued after
+    // b's wait would also wait for it
+    queue_probe_kernel<<<1, 64, 0, a>>>(ticks);
+    if (hipEventRecord(e[1], a) != hipSuccess) rc = PPSK_ERR_HIP;
+    queue_probe_kernel<<<1, 64, 0, b>>>(ticks);
+    if (rc != PPSK_OK || hipGetLastError() != hipSuccess ||
+        hipEventRecord(e[2], b) != hipSuccess || hipStreamSynchronize(a) != hipSuccess ||
+        hipStreamSynchronize(b) != hipSuccess || hipEventElapsedTime(&ta, e[0], e[1]) != hipSuccess ||
+        hipEventElapsedTime(&tb, e[0], e[2]) != hipSuccess) {
+      rc = PPSK_ERR_HIP;
+    }
+  }
+  for (auto& x : e) {
+    if (x) hipEventDestroy(x);
+  }
+  // ta: one wait; tb: one wait when concurrent, two when serialised
+  *concurrent = rc == PPSK_OK && tb < 1.5f * ta;
+  return rc;
+}
+
+// n streams for the serving slots, the first few placed on distinct hardware
+// queues where the runtime allows: a new stream that runs serialised with an
+// earlier one is kept aside (so that its queue counts as busier) and another
+// is created, at most three times per slot; the streams kept aside are then
+// destroyed.  Slots beyond the first four share queues in any case.
+int create_slot_streams(int n, std::vector<hipStream_t>* out) {
+  const int ticks = 600;
+  std::vector<hipStream_t> aside;
+  int rc = PPSK_OK;
+  for (int i = 0; i < n && rc == PPSK_OK; ++i) {
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      rc = PPSK_ERR_HIP;
+      break;
+    }
+    for (int tries = 0; i > 0 && i < 4 && tries < 3; ++tries) {
+      bool all = true;
+      for (hipStream_t o : *out) {
+        bool c = false;
+        if ((rc = streams_concurrent(o, st, ticks, &c)) != PPSK_OK) break;
+        all = all && c;
+      }
+      if (rc != PPSK_OK || all) break;
+      aside.push_back(st);
+      st = nullptr;
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        rc = PPSK_ERR_HIP;
+        break;
+      }
+    }
+    if (st) out->push_back(st);
+  }
+  for (hipStream_t st : aside) hipStreamDestroy(st);
+  return rc;
+}
+
 void worker_main(ppsk_server* s, Worker* w) {
   hipSetDevice(s->device);
   for (;;) {
@@ -614,6 +695,7 @@ int ppsk_server_create(ppsk_set* ref, int32_t random_mode, double tolerance, int
   }
   DeviceCtx* c = nullptr;
   if (int rc = get_ctx(ref->device, &c)) return rc;
+  if (hipSetDevice(ref->device) != hipSuccess) return fail(PPSK_ERR_HIP, "hipSetDevice failed");
   auto* s = new ppsk_server();
   s->ref = ref;
   s->device = ref->device;
@@ -651,10 +733,18 @@ int ppsk_server_create(ppsk_set* ref, int32_t random_mode, double tolerance, int
   }
   s->arena.init((char*)s->arena_mem, arena_bytes);
   s->t0 = std::chrono::steady_clock::now();
+  std::vector<hipStream_t> streams;
+  if (create_slot_streams(n_slots, &streams) != PPSK_OK) {
+    for (hipStream_t st : streams) hipStreamDestroy(st);
+    ppsk_server_free(s);
+    return fail(PPSK_ERR_HIP, "serving stream creation");
+  }
   for (int i = 0; i < n_slots; ++i) {
     auto* w = new Worker();
-    if (hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&w->e0) != hipSuccess || hipEventCreate(&w->e1) != hipSuccess) {
+    w->st = streams[i];
+    if (hipEventCreate(&w->e0) != hipSuccess || hipEventCreate(&w->e1) != hipSuccess) {
+      for (int x = i; x < n_slots; ++x) hipStreamDestroy(streams[x]);
+      if (w->e0) hipEventDestroy(w->e0);
       delete w;
       ppsk_server_free(s);
       return fail(PPSK_ERR_HIP, "serving stream creation");
