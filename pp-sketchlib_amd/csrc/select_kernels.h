@@ -21,25 +21,36 @@ hipError_t launch_knn_select_triangle(const float* tri, int64_t n, int dist_col,
                                       hipStream_t stream);
 
 // Query rows against a separate reference set: band [n_rows][n] float2,
-// nothing excluded; i_vec holds the band row.
+// nothing excluded; i_vec holds the band row.  segs > 1: each row is cut into
+// segs column segments that list their candidates in parallel before one wave
+// per row replays the heap over the lists (same result), for steps of few
+// rows; the lists live in `workspace` (knn_rect_workspace bytes).
 hipError_t launch_knn_select_rect(const float* band, int64_t n_rows, int64_t n, int dist_col,
                                   int kNN, int64_t* i_vec, int64_t* j_vec, float* dists,
-                                  hipStream_t stream);
+                                  hipStream_t stream, int segs = 1, void* workspace = nullptr);
+// Candidate list capacity per (row, segment); a longer list falls back to a
+// direct scan of its segment in the merge.
+int knn_rect_capacity(int kNN);
+size_t knn_rect_workspace(int64_t n_rows, int segs, int kNN);
 
-// Threshold in two passes over `rows` rows: per-row counts of d < cutoff,
-// then (after an exclusive scan of the counts into offsets) the entries in
-// row-major order.  rect: rows of n entries of a query-major band (j = column);
+// Threshold in two passes over `rows` rows: counts of d < cutoff, then (after
+// an exclusive scan of the counts into offsets) the entries in row-major
+// order.  rect: rows of n entries of a query-major band (j = column);
 // otherwise the condensed self triangle of n samples (rows = n - 1, j > i).
-// Outputs may be device or page-locked host memory.
+// Each row is cut into `segs` equal column segments, one workgroup each, with
+// one count / offset per (row, segment) in row-major order (rows * segs
+// entries), so that a few long rows still fill the GPU.  Outputs may be
+// device or page-locked host memory.
 hipError_t launch_threshold_count(const float2* dd, int64_t rows, int64_t n, bool rect,
                                   int dist_col, float cutoff, int64_t* counts,
-                                  hipStream_t stream);
+                                  hipStream_t stream, int segs = 1);
 // Entries at offsets >= capacity are not written (the caller sizes the
 // outputs from the total and writes again).
 hipError_t launch_threshold_write(const float2* dd, int64_t rows, int64_t n, bool rect,
                                   int dist_col, float cutoff, const int64_t* offsets,
                                   int64_t* i_vec, int64_t* j_vec, float* dists,
-                                  hipStream_t stream, int64_t capacity = INT64_MAX);
+                                  hipStream_t stream, int64_t capacity = INT64_MAX,
+                                  int segs = 1);
 // Per-row counts -> exclusive offsets in place, total in counts[rows]
 // (device memory of rows + 1 entries; one workgroup).
 hipError_t launch_offsets_scan(int64_t* counts, int64_t rows, hipStream_t stream);

@@ -148,6 +148,151 @@ __global__ __launch_bounds__(64) void knn_select_kernel(Rows rows, int64_t r0, i
   }
 }
 
+// ---------------------------------------------------------------------------
+// kNN of a few long query rows (serving): two passes, so that a step of a
+// hundred rows still fills the GPU.
+//
+// 1. knn_cand_kernel, one wave per (row, column segment): the same scan,
+//    started empty at the segment's first column, lists every element it
+//    admits (column order).  An element the full scan admits is below the
+//    k-th smallest of all earlier columns of its row (or the heap is not yet
+//    full), and that is at most the k-th smallest of the earlier columns of
+//    its segment, so the segment scan admits it too: the lists hold every
+//    element the full scan admits.
+// 2. knn_merge_kernel, one wave per row: the full scan's heap, replayed over
+//    the lists of segments 0, 1, ... in column order.  Elements missing from
+//    the lists are ones the full scan rejects without touching the heap, so
+//    the heap goes through the same pushes and pops and ends the same.  A
+//    segment whose list overflowed its capacity is scanned directly.
+// ---------------------------------------------------------------------------
+
+// Offers one chunk (lane t holds d, j; valid lanes only) to the heap in lane
+// order, admitting as the scan does; lane 0 calls on_admit(d, j, base + x)
+// for the x-th admitted element.  Returns the number admitted (wave-uniform).
+template <typename F>
+__device__ __forceinline__ int offer_chunk(HeapEnt* heap, int kNN, int& len, float& top,
+                                           bool valid, float d, int32_t j, int lane, int base,
+                                           F on_admit) {
+  unsigned long long mask = __ballot(valid && (len < kNN || d < top));
+  int x = 0;
+  while (mask) {
+    const int b = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    const float db = __shfl(d, b);
+    const int32_t jb = __shfl(j, b);
+    if (len < kNN || db < top) {
+      float new_top = 0.0f;
+      if (lane == 0) {
+        heap_push(heap, len, HeapEnt{db, jb});
+        if (len + 1 > kNN) heap_pop(heap, len + 1);
+        new_top = heap[0].d;
+        on_admit(db, jb, base + x);
+      }
+      ++x;
+      len = len < kNN ? len + 1 : kNN;
+      new_top = __shfl(new_top, 0);
+      if (len == kNN) top = new_top;
+    }
+  }
+  return x;
+}
+
+// Scan of columns [lo, hi) of row `row` of a float2 band (rows of n), the next
+// chunk loaded before the current one is offered.  Returns the number of
+// admitted elements; on_admit as in offer_chunk, numbered from `base`.
+template <typename F>
+__device__ __forceinline__ int scan_columns(const float* band, int64_t row, int64_t n, int col,
+                                            int64_t lo, int64_t hi, HeapEnt* heap, int kNN,
+                                            int& len, float& top, int lane, int base,
+                                            F on_admit) {
+  auto load = [&](int64_t c) {
+    const int64_t jj = c + lane;
+    return jj < hi ? band[(row * n + jj) * 2 + col] : INFINITY;
+  };
+  int count = 0;
+  float d_next = load(lo);
+  for (int64_t c0 = lo; c0 < hi; c0 += 64) {
+    const float d = d_next;
+    d_next = load(c0 + 64);
+    count += offer_chunk(heap, kNN, len, top, c0 + lane < hi, d, (int32_t)(c0 + lane), lane,
+                         base + count, on_admit);
+  }
+  return count;
+}
+
+// Writes the kNN of row `row` from its heap (len == kNN), largest last, as the
+// reference drains its priority_queue.
+__device__ __forceinline__ void drain_heap(HeapEnt* heap, int kNN, int len, int64_t row,
+                                           int64_t gi, int64_t* iv, int64_t* jv, float* dv) {
+  const int64_t off = row * kNN;
+  int l = len;
+  for (int k = kNN - 1; k >= 0; --k) {
+    iv[off + k] = gi;
+    jv[off + k] = heap[0].j;
+    dv[off + k] = heap[0].d;
+    heap_pop(heap, l);
+    --l;
+  }
+}
+
+__global__ __launch_bounds__(64) void knn_cand_kernel(const float* __restrict__ band, int64_t n,
+                                                      int col, int kNN, int segs, int cap,
+                                                      HeapEnt* __restrict__ cand,
+                                                      int32_t* __restrict__ cand_n) {
+  extern __shared__ HeapEnt heap[];
+  const int64_t b = blockIdx.x;
+  const int64_t row = b / segs, g = b % segs;
+  const int64_t lo = n * g / segs, hi = n * (g + 1) / segs;
+  const int lane = threadIdx.x;
+  HeapEnt* out = cand + b * cap;
+  int len = 0;
+  float top = 0.0f;
+  const int count = scan_columns(band, row, n, col, lo, hi, heap, kNN, len, top, lane, 0,
+                                 [&](float d, int32_t j, int x) {
+                                   if (x < cap) out[x] = HeapEnt{d, j};
+                                 });
+  if (lane == 0) cand_n[b] = count;  // > cap: the list overflowed
+}
+
+__global__ __launch_bounds__(64) void knn_merge_kernel(const float* __restrict__ band, int64_t n,
+                                                       int col, int kNN, int segs, int cap,
+                                                       const HeapEnt* __restrict__ cand,
+                                                       const int32_t* __restrict__ cand_n,
+                                                       int64_t* __restrict__ iv,
+                                                       int64_t* __restrict__ jv,
+                                                       float* __restrict__ dv) {
+  extern __shared__ HeapEnt heap[];
+  const int64_t row = blockIdx.x;
+  const int lane = threadIdx.x;
+  int len = 0;
+  float top = 0.0f;
+  auto none = [](float, int32_t, int) {};
+  for (int g = 0; g < segs; ++g) {
+    const int64_t b = row * segs + g;
+    const int cnt = cand_n[b];
+    if (cnt <= cap) {
+      const HeapEnt* in = cand + b * cap;
+      for (int c0 = 0; c0 < cnt; c0 += 64) {
+        const int t = c0 + lane;
+        const HeapEnt e = t < cnt ? in[t] : HeapEnt{INFINITY, 0};
+        offer_chunk(heap, kNN, len, top, t < cnt, e.d, e.j, lane, 0, none);
+      }
+    } else {
+      const int64_t lo = n * g / segs, hi = n * (g + 1) / segs;
+      scan_columns(band, row, n, col, lo, hi, heap, kNN, len, top, lane, 0, none);
+    }
+  }
+  if (lane == 0) drain_heap(heap, kNN, len, row, row, iv, jv, dv);
+}
+
+int knn_rect_capacity(int kNN) { return 16 * kNN + 64; }
+
+size_t knn_rect_workspace(int64_t n_rows, int segs, int kNN) {
+  if (segs <= 1) return 0;
+  const size_t lists = (size_t)n_rows * segs;
+  return lists * knn_rect_capacity(kNN) * sizeof(HeapEnt) + lists * sizeof(int32_t);
+}
+
 hipError_t launch_knn_select(const float* band, int64_t r0, int64_t n_rows, int64_t n,
                              int dist_col, int kNN, int64_t* i_vec, int64_t* j_vec,
                              float* dists, hipStream_t stream) {
@@ -161,11 +306,24 @@ hipError_t launch_knn_select(const float* band, int64_t r0, int64_t n_rows, int6
 
 hipError_t launch_knn_select_rect(const float* band, int64_t n_rows, int64_t n, int dist_col,
                                   int kNN, int64_t* i_vec, int64_t* j_vec, float* dists,
-                                  hipStream_t stream) {
+                                  hipStream_t stream, int segs, void* workspace) {
   if (kNN < 1 || kNN > KNN_MAX || kNN > n) return hipErrorInvalidValue;
   if (n_rows <= 0) return hipSuccess;
-  knn_select_kernel<BandRows><<<(unsigned)n_rows, 64, (kNN + 1) * sizeof(HeapEnt), stream>>>(
-      BandRows{band, 0, n, dist_col}, 0, n, kNN, false, i_vec, j_vec, dists);
+  const size_t heap_bytes = (kNN + 1) * sizeof(HeapEnt);
+  if (segs <= 1) {
+    knn_select_kernel<BandRows><<<(unsigned)n_rows, 64, heap_bytes, stream>>>(
+        BandRows{band, 0, n, dist_col}, 0, n, kNN, false, i_vec, j_vec, dists);
+    return hipGetLastError();
+  }
+  if (!workspace || n_rows * segs > INT32_MAX || segs > n) return hipErrorInvalidValue;
+  const int cap = knn_rect_capacity(kNN);
+  HeapEnt* cand = (HeapEnt*)workspace;
+  int32_t* cand_n = (int32_t*)(cand + (size_t)n_rows * segs * cap);
+  knn_cand_kernel<<<(unsigned)(n_rows * segs), 64, heap_bytes, stream>>>(band, n, dist_col, kNN,
+                                                                          segs, cap, cand, cand_n);
+  knn_merge_kernel<<<(unsigned)n_rows, 64, heap_bytes, stream>>>(band, n, dist_col, kNN, segs, cap,
+                                                                  cand, cand_n, i_vec, j_vec,
+                                                                  dists);
   return hipGetLastError();
 }
 
@@ -197,13 +355,18 @@ __device__ __forceinline__ RowSpan row_span(bool rect, int64_t i, int64_t n) {
   return RowSpan{n * i - ((i * (i + 1)) >> 1), n - 1 - i, i + 1};
 }
 
+// Block b covers segment b % segs of row b / segs: entries [lo, hi) of the
+// row, lo = len * g / segs, hi = len * (g + 1) / segs; counts and offsets are
+// per (row, segment) in row-major order, so the output order is row-major.
 __global__ __launch_bounds__(TB) void thr_count_kernel(const float2* __restrict__ dd, int64_t n,
                                                        bool rect, int col, float cutoff,
-                                                       int64_t* __restrict__ counts) {
-  const int64_t i = blockIdx.x;
+                                                       int64_t* __restrict__ counts, int segs) {
+  const int64_t b = blockIdx.x;
+  const int64_t i = b / segs, g = b % segs;
   const RowSpan sp = row_span(rect, i, n);
+  const int64_t lo = sp.len * g / segs, hi = sp.len * (g + 1) / segs;
   int64_t c = 0;
-  for (int64_t t = threadIdx.x; t < sp.len; t += TB) {
+  for (int64_t t = lo + threadIdx.x; t < hi; t += TB) {
     c += pick(dd[sp.start + t], col) < cutoff ? 1 : 0;
   }
   __shared__ int64_t red[TB / 64];
@@ -214,7 +377,7 @@ __global__ __launch_bounds__(TB) void thr_count_kernel(const float2* __restrict_
   if (threadIdx.x == 0) {
     int64_t s = 0;
     for (int w = 0; w < TB / 64; ++w) s += red[w];
-    counts[i] = s;
+    counts[b] = s;
   }
 }
 
@@ -257,20 +420,23 @@ __global__ __launch_bounds__(TB) void thr_write_kernel(const float2* __restrict_
                                                        const int64_t* __restrict__ offsets,
                                                        int64_t* __restrict__ iv,
                                                        int64_t* __restrict__ jv,
-                                                       float* __restrict__ dv, int64_t cap) {
-  const int64_t i = blockIdx.x;
+                                                       float* __restrict__ dv, int64_t cap,
+                                                       int segs) {
+  const int64_t b = blockIdx.x;
+  const int64_t i = b / segs, g = b % segs;
   const RowSpan sp = row_span(rect, i, n);
+  const int64_t lo = sp.len * g / segs, hi = sp.len * (g + 1) / segs;
   __shared__ int wave_tot[TB / 64];
   __shared__ int64_t base_sh;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) base_sh = offsets[i];
+  if (threadIdx.x == 0) base_sh = offsets[b];
   __syncthreads();
-  for (int64_t c0 = 0; c0 < sp.len; c0 += TB) {
+  for (int64_t c0 = lo; c0 < hi; c0 += TB) {
     const int64_t t = c0 + threadIdx.x;
     float d = 0.0f;
     bool keep = false;
-    if (t < sp.len) {
+    if (t < hi) {
       d = pick(dd[sp.start + t], col);
       keep = d < cutoff;
     }
@@ -299,19 +465,23 @@ __global__ __launch_bounds__(TB) void thr_write_kernel(const float2* __restrict_
 
 hipError_t launch_threshold_count(const float2* dd, int64_t rows, int64_t n, bool rect,
                                   int dist_col, float cutoff, int64_t* counts,
-                                  hipStream_t stream) {
+                                  hipStream_t stream, int segs) {
   if (rows <= 0) return hipSuccess;
-  thr_count_kernel<<<(unsigned)rows, TB, 0, stream>>>(dd, n, rect, dist_col, cutoff, counts);
+  if (segs < 1 || rows * segs > INT32_MAX) return hipErrorInvalidValue;
+  thr_count_kernel<<<(unsigned)(rows * segs), TB, 0, stream>>>(dd, n, rect, dist_col, cutoff,
+                                                               counts, segs);
   return hipGetLastError();
 }
 
 hipError_t launch_threshold_write(const float2* dd, int64_t rows, int64_t n, bool rect,
                                   int dist_col, float cutoff, const int64_t* offsets,
                                   int64_t* i_vec, int64_t* j_vec, float* dists,
-                                  hipStream_t stream, int64_t capacity) {
+                                  hipStream_t stream, int64_t capacity, int segs) {
   if (rows <= 0) return hipSuccess;
-  thr_write_kernel<<<(unsigned)rows, TB, 0, stream>>>(dd, n, rect, dist_col, cutoff, offsets,
-                                                       i_vec, j_vec, dists, capacity);
+  if (segs < 1 || rows * segs > INT32_MAX) return hipErrorInvalidValue;
+  thr_write_kernel<<<(unsigned)(rows * segs), TB, 0, stream>>>(dd, n, rect, dist_col, cutoff,
+                                                               offsets, i_vec, j_vec, dists,
+                                                               capacity, segs);
   return hipGetLastError();
 }
 

@@ -181,8 +181,10 @@ int step_variant(const ppsk_server* s, int64_t rows) {
 // launched in chunks of chunk_tiles).
 struct StepLayout {
   size_t staging, stripes, jv, band, tiles, scratch, tile_done, fail, fail_rows, params;
-  size_t tail_tiles, tail_counts, tail_done, counts, total;
+  size_t tail_tiles, tail_counts, tail_done, counts, knn_lists, total;
   int64_t nbq, n_tiles, chunk_tiles, tail;
+  int segs;      // column segments per row of the threshold kernels
+  int knn_segs;  // column segments per row of the kNN candidate lists
   bool persistent;
   int grid;
 };
@@ -228,7 +230,13 @@ StepLayout layout(const ppsk_server* s, int64_t rows) {
   }
   L.fail = take(2 * sizeof(unsigned long long));
   L.fail_rows = take((size_t)rows * sizeof(uint32_t));
-  L.counts = take((size_t)(rows + 1) * sizeof(int64_t));
+  // about 2,048 threshold workgroups, segments of at least 4,096 entries
+  L.segs = (int)std::max<int64_t>(1, std::min<int64_t>((2048 + rows - 1) / rows, r->n / 4096));
+  L.counts = take((size_t)(rows * L.segs + 1) * sizeof(int64_t));
+  // the same segments list kNN candidates (small kNN only: the lists grow
+  // with it)
+  L.knn_segs = s->kNN > 0 && s->kNN <= 64 ? L.segs : 1;
+  L.knn_lists = take(ppsk::knn_rect_workspace(rows, L.knn_segs, std::max(1, s->kNN)));
   L.total = off;
   return L;
 }
@@ -409,9 +417,11 @@ int run_step(ppsk_server* s, Worker& w, std::vector<Piece>& batch, int64_t rows,
     }
     STEP_TRY(ppsk::launch_knn_select_rect(band, rows, ref->n, s->knn_col, s->kNN,
                                           (int64_t*)w.knn_i.p, (int64_t*)w.knn_j.p,
-                                          (float*)w.knn_d.p, w.st));
+                                          (float*)w.knn_d.p, w.st, L.knn_segs,
+                                          mem + L.knn_lists));
   }
-  if (!w.counts.reserve((size_t)(rows + 1) * 8) || !w.fail_rows.reserve((size_t)rows * 4)) {
+  const int64_t nseg = rows * L.segs;  // (row, segment) counts, row-major
+  if (!w.counts.reserve((size_t)(nseg + 1) * 8) || !w.fail_rows.reserve((size_t)rows * 4)) {
     err = "page-locked count buffers";
     return PPSK_ERR_MEMORY;
   }
@@ -431,18 +441,19 @@ int run_step(ppsk_server* s, Worker& w, std::vector<Piece>& batch, int64_t rows,
     }
     cap = (int64_t)std::min({w.thr_i.n / 8, w.thr_j.n / 8, w.thr_d.n / 4});
     STEP_TRY(ppsk::launch_threshold_count((const float2*)band, rows, ref->n, true, s->thr_col,
-                                          s->cutoff, dcnt, w.st));
-    STEP_TRY(ppsk::launch_offsets_scan(dcnt, rows, w.st));
+                                          s->cutoff, dcnt, w.st, L.segs));
+    STEP_TRY(ppsk::launch_offsets_scan(dcnt, nseg, w.st));
     STEP_TRY(ppsk::launch_threshold_write((const float2*)band, rows, ref->n, true, s->thr_col,
                                           s->cutoff, dcnt, (int64_t*)w.thr_i.p,
-                                          (int64_t*)w.thr_j.p, (float*)w.thr_d.p, w.st, cap));
-    STEP_TRY(hipMemcpyAsync(cnt, dcnt, (size_t)(rows + 1) * 8, hipMemcpyDeviceToHost, w.st));
+                                          (int64_t*)w.thr_j.p, (float*)w.thr_d.p, w.st, cap,
+                                          L.segs));
+    STEP_TRY(hipMemcpyAsync(cnt, dcnt, (size_t)(nseg + 1) * 8, hipMemcpyDeviceToHost, w.st));
   }
   STEP_TRY(hipMemcpyAsync(w.fail_rows.p, fail_rows, (size_t)rows * 4, hipMemcpyDeviceToHost, w.st));
   STEP_TRY(hipStreamSynchronize(w.st));
   float ms = 0.f;
   STEP_TRY(hipEventElapsedTime(&ms, w.e0, w.e1));
-  const int64_t total = s->cutoff > 0 ? cnt[rows] : 0;
+  const int64_t total = s->cutoff > 0 ? cnt[nseg] : 0;
   if (total > cap) {
     if (!w.thr_i.reserve((size_t)total * 8) || !w.thr_j.reserve((size_t)total * 8) ||
         !w.thr_d.reserve((size_t)total * 4)) {
@@ -451,7 +462,8 @@ int run_step(ppsk_server* s, Worker& w, std::vector<Piece>& batch, int64_t rows,
     }
     STEP_TRY(ppsk::launch_threshold_write((const float2*)band, rows, ref->n, true, s->thr_col,
                                           s->cutoff, dcnt, (int64_t*)w.thr_i.p,
-                                          (int64_t*)w.thr_j.p, (float*)w.thr_d.p, w.st));
+                                          (int64_t*)w.thr_j.p, (float*)w.thr_d.p, w.st,
+                                          INT64_MAX, L.segs));
     STEP_TRY(hipStreamSynchronize(w.st));
   }
 
@@ -474,7 +486,7 @@ int run_step(ppsk_server* s, Worker& w, std::vector<Piece>& batch, int64_t rows,
       std::copy(sd + a, sd + b, r->kd.begin() + at);
     }
     if (s->cutoff > 0 && total > 0) {
-      const int64_t a = cnt[s0], b = cnt[s1];
+      const int64_t a = cnt[s0 * L.segs], b = cnt[s1 * L.segs];
       const int64_t* si = (const int64_t*)w.thr_i.p;
       pc.thr.i.resize(b - a);
       for (int64_t x = a; x < b; ++x) pc.thr.i[x - a] = si[x] + shift;

@@ -198,7 +198,9 @@ def test_persistent_step_tail_split(monkeypatch):
     # 33,000 references: a 512-query step is 8 x 129 tiles of 64 x 256 on the
     # persistent kernel, two full rounds of its 512 resident workgroups and 8
     # tiles more, which run as (tile, k-mer length) items after it; results
-    # equal the step without that split and the plain path
+    # equal the step without that split and the plain path.  The threshold
+    # kernels cut each row into 4 column segments here (8 for the 40-query
+    # request below)
     n_ref, n_q = 33_000, 512
     # core divergence up to 15% keeps the 0.05 threshold to close relatives
     raw, le, _ = H.synth(n_ref + n_q, KLIST8, S64, seed=23, dcore_max=0.15)
@@ -213,13 +215,17 @@ def test_persistent_step_tail_split(monkeypatch):
                 res[split] = srv.wait(srv.submit(q_raw, q_le))
             assert res[split]["n_steps"] == 1
         _same(res["1"], res["0"])
-        _, nf, (ki, kj, kd), (ti, tj, td) = _plain(ref, q_raw, q_le)
-    r = res["1"]
-    assert r["n_fail"] == nf
-    assert np.array_equal(r["knn_i"], ki) and np.array_equal(r["knn_j"], kj)
-    assert np.array_equal(r["knn_d"].view(np.uint32), kd.view(np.uint32))
-    assert np.array_equal(r["thr_i"], ti) and np.array_equal(r["thr_j"], tj)
-    assert np.array_equal(r["thr_d"].view(np.uint32), td.view(np.uint32))
+        with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
+                      max_rows=n_q, n_slots=1) as srv:
+            small = srv.wait(srv.submit(q_raw[100:140], q_le[100:140]))
+        for r, (s, e) in ((res["1"], (0, n_q)), (small, (100, 140))):
+            _, nf, (ki, kj, kd), (ti, tj, td) = _plain(ref, q_raw[s:e], q_le[s:e])
+            assert r["n_fail"] == nf
+            assert np.array_equal(r["knn_i"], ki) and np.array_equal(r["knn_j"], kj)
+            assert np.array_equal(r["knn_d"].view(np.uint32), kd.view(np.uint32))
+            assert len(ti) > 0
+            assert np.array_equal(r["thr_i"], ti) and np.array_equal(r["thr_j"], tj)
+            assert np.array_equal(r["thr_d"].view(np.uint32), td.view(np.uint32))
 
 
 def test_threshold_hits_beyond_buffer_capacity(data):
@@ -240,3 +246,27 @@ def test_threshold_hits_beyond_buffer_capacity(data):
         assert len(ti) > 65536
         assert np.array_equal(r["thr_i"], ti) and np.array_equal(r["thr_j"], tj)
         assert np.array_equal(r["thr_d"].view(np.uint32), core[ti, tj].view(np.uint32))
+
+
+def test_knn_candidate_lists_overflow():
+    # 33,000 references ordered by decreasing core distance to the query: a
+    # one-query step cuts its row into 8 column segments, each of which admits
+    # every one of its ~4,000 columns, far more than a candidate list holds,
+    # so the merge scans those segments directly; the kNN still equals the
+    # reference heap over the whole row (ties included)
+    n_ref = 33_000
+    raw, le, _ = H.synth(n_ref + 1, KLIST8, S64, seed=29, dcore_max=0.15)
+    q_raw, q_le = raw[n_ref:].copy(), le[n_ref:].copy()
+    with H.SketchSet(raw[:n_ref], KLIST8, S64, lengths=le[:n_ref]) as ref0, \
+            H.SketchSet(q_raw, KLIST8, S64, lengths=q_le) as q:
+        d0, _, _ = H.dense(ref0, q, random_mode=H.RANDOM_EQUAL_BASE)
+    order = np.argsort(-d0[:, 0], kind="stable")
+    with H.SketchSet(raw[:n_ref][order], KLIST8, S64, lengths=le[:n_ref][order]) as ref:
+        del raw
+        with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
+                      max_rows=64, n_slots=1) as srv:
+            r = srv.wait(srv.submit(q_raw, q_le))
+        d, nf, (ki, kj, kd), _ = _plain(ref, q_raw, q_le)
+    assert np.all(np.diff(d[:, 0]) <= 0)  # non-increasing along the row
+    assert np.array_equal(r["knn_i"], ki) and np.array_equal(r["knn_j"], kj)
+    assert np.array_equal(r["knn_d"].view(np.uint32), kd.view(np.uint32))
