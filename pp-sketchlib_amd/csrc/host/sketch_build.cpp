@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iostream>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <sys/stat.h>
@@ -44,16 +45,79 @@ struct SampleSeq {
   bool reads = false;
 };
 
+// The whole (possibly gzipped) file, read straight into a buffer sized from
+// the file (doubled as a compressed file inflates).  A plain file (no gzip
+// magic bytes) is read with one fread.
 std::string read_all(const std::string& path) {
+  if (FILE* fh = std::fopen(path.c_str(), "rb")) {
+    unsigned char magic[2] = {0, 0};
+    const size_t m = std::fread(magic, 1, 2, fh);
+    long size = -1;
+    if (!(m == 2 && magic[0] == 0x1f && magic[1] == 0x8b) && std::fseek(fh, 0, SEEK_END) == 0) {
+      size = std::ftell(fh);
+    }
+    if (size >= 0 && std::fseek(fh, 0, SEEK_SET) == 0) {
+      std::string buf((size_t)size, '\0');
+      const size_t got = size > 0 ? std::fread(&buf[0], 1, (size_t)size, fh) : 0;
+      const bool err = std::ferror(fh) != 0;
+      std::fclose(fh);
+      if (err) throw std::runtime_error("Could not read " + path);
+      buf.resize(got);
+      return buf;
+    }
+    std::fclose(fh);
+  }
   gzFile fp = gzopen(path.c_str(), "r");
   if (!fp) throw std::runtime_error("Could not open " + path);
-  std::string buf;
-  std::vector<char> chunk(1 << 20);
-  int n;
-  while ((n = gzread(fp, chunk.data(), (unsigned)chunk.size())) > 0) buf.append(chunk.data(), n);
+  gzbuffer(fp, 1 << 20);
+  struct stat st;
+  size_t cap = stat(path.c_str(), &st) == 0 ? (size_t)st.st_size + 1 : (size_t)1 << 20;
+  std::string buf(std::max<size_t>(cap, 1 << 16), '\0');
+  size_t len = 0;
+  int n = 0;
+  for (;;) {
+    if (len == buf.size()) buf.resize(buf.size() * 2);
+    const size_t want = std::min<size_t>(buf.size() - len, (size_t)1 << 30);
+    n = gzread(fp, &buf[len], (unsigned)want);
+    if (n <= 0) break;
+    len += (size_t)n;
+  }
   gzclose(fp);
   if (n < 0) throw std::runtime_error("Could not read " + path);
+  buf.resize(len);
   return buf;
+}
+
+// Upper-cases a kept contig in place and adds it to the sample's counts:
+// A, C, G, T/U over the sample's first 999,999 bases (the reference's counter
+// is incremented before its test against 1,000,000, so a base at index x of
+// this contig counts while total + x + 1 < 1,000,000), N over every base.
+void add_contig(std::string& seq, SampleSeq& out) {
+  char* p = &seq[0];
+  const size_t n = seq.size();
+  for (size_t x = 0; x < n; ++x) {
+    const char ch = p[x];
+    p[x] = ('a' <= ch && ch <= 'z') ? (char)(ch ^ 0x20) : ch;
+  }
+  const size_t lim = out.total + 1 < MAX_COMPOSITION_SAMPLE
+                         ? std::min(n, MAX_COMPOSITION_SAMPLE - 1 - out.total)
+                         : 0;
+  size_t a = 0, c = 0, g = 0, t = 0;
+  for (size_t x = 0; x < lim; ++x) {
+    const char ch = p[x];
+    a += ch == 'A';
+    c += ch == 'C';
+    g += ch == 'G';
+    t += (ch == 'T') | (ch == 'U');
+  }
+  size_t nn = 0;
+  for (size_t x = 0; x < n; ++x) nn += p[x] == 'N';
+  out.a += a;
+  out.c += c;
+  out.g += g;
+  out.t += t;
+  out.n_count += nn;
+  out.total += n;
 }
 
 // Appends the line starting at i (without its '\n') to `to`, dropping a
@@ -99,31 +163,12 @@ void read_file(const std::string& path, size_t kmer_len, SampleSeq& out) {
       if (q.size() != seq.size()) return;  // kseq: a quality of another length ends the file
       qual = !q.empty();
     }
-    seq.resize(std::strlen(seq.c_str()));  // the reader keeps the C string (seqio.cpp:68-73)
-    const size_t seq_len = seq.size();
-    if (seq_len >= kmer_len) {
-      for (char& ch : seq) {
-        if ('a' <= ch && ch <= 'z') ch = (char)(ch ^ 0x20);
-        out.total++;
-        if (out.total < MAX_COMPOSITION_SAMPLE) {
-          switch (ch) {
-            case 'A':
-              out.a++;
-              break;
-            case 'C':
-              out.c++;
-              break;
-            case 'G':
-              out.g++;
-              break;
-            case 'T':
-            case 'U':
-              out.t++;
-              break;
-          }
-        }
-        if (ch == 'N') out.n_count++;
-      }
+    // the reader keeps the C string (seqio.cpp:68-73): up to a NUL
+    if (const void* z = std::memchr(seq.data(), 0, seq.size())) {
+      seq.resize((size_t)((const char*)z - seq.data()));
+    }
+    if (seq.size() >= kmer_len) {
+      add_contig(seq, out);
       out.contigs.push_back(std::move(seq));
     }
     if (qual) out.reads = true;
@@ -243,21 +288,27 @@ std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::s
       bases += seqs[s1].total;
       ++s1;
     }
-    std::string seq;
-    seq.reserve(bases);
+    // the batch's contigs back to back: offsets first, then the copies in
+    // parallel into an uninitialised buffer
     std::vector<int64_t> off{0}, sc{0};
+    std::vector<size_t> first_contig;
     for (size_t s = s0; s < s1; ++s) {
-      for (const auto& c : seqs[s].contigs) {
-        seq += c;
-        off.push_back((int64_t)seq.size());
-      }
+      first_contig.push_back(off.size() - 1);
+      for (const auto& c : seqs[s].contigs) off.push_back(off.back() + (int64_t)c.size());
       sc.push_back((int64_t)off.size() - 1);
+    }
+    const size_t seq_len = (size_t)off.back();
+    std::unique_ptr<char[]> seq(new char[std::max<size_t>(seq_len, 1)]);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(std::max<size_t>(num_threads, 1))
+    for (long x = 0; x < (long)(s1 - s0); ++x) {
+      size_t ci = first_contig[x];
+      for (const auto& c : seqs[s0 + x].contigs) std::memcpy(seq.get() + off[ci++], c.data(), c.size());
     }
     const int64_t n = (int64_t)(s1 - s0);
     std::vector<uint64_t> raw((size_t)n * K * words);
     std::vector<double> mh(n);
     std::vector<int32_t> dn(n);
-    const int rc = ppsk_sketch_ex(device_id, n, seq.data(), (int64_t)seq.size(), off.data(),
+    const int rc = ppsk_sketch_ex(device_id, n, seq.get(), (int64_t)seq_len, off.data(),
                                   (int64_t)off.size() - 1, sc.data(), (int32_t)K, kl.data(),
                                   (int32_t)sketchsize64, use_rc ? 1 : 0, codon_phased ? 1 : 0,
                                   filtered(s0) ? mc : 0, exact ? 1 : 0, raw.data(), mh.data(),

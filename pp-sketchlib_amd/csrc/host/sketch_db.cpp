@@ -29,9 +29,23 @@ struct H5Handle {
 
 void quiet() { H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr); }
 
+// File access for writing sketches: a 4 MiB sieve buffer gathers the many
+// small dataset writes, and metadata and small raw data are allocated from
+// 1 MiB blocks; this changes where objects sit in the file, not what it holds
+// (64 samples of 8 k-mer lengths: about 15 ms instead of 18-26 ms).
+H5Handle write_access() {
+  H5Handle fapl(H5Pcreate(H5P_FILE_ACCESS), H5Pclose);
+  H5Pset_sieve_buf_size(fapl, (size_t)4 << 20);
+  H5Pset_meta_block_size(fapl, (hsize_t)1 << 20);
+  H5Pset_small_data_block_size(fapl, (hsize_t)1 << 20);
+  return fapl;
+}
+
 H5Handle open_file(const std::string& path, bool write) {
   quiet();
-  hid_t f = H5Fopen(path.c_str(), write ? H5F_ACC_RDWR : H5F_ACC_RDONLY, H5P_DEFAULT);
+  H5Handle fapl = write ? write_access() : H5Handle(H5P_DEFAULT, nullptr);
+  hid_t f = H5Fopen(path.c_str(), write ? H5F_ACC_RDWR : H5F_ACC_RDONLY,
+                    write ? (hid_t)fapl : H5P_DEFAULT);
   if (f < 0) throw HDF5Error("Unable to open file " + path);
   return H5Handle(f, H5Fclose);
 }
@@ -326,7 +340,8 @@ RandomTables read_random(const std::string& h5_path) {
 void write_new_db(const std::string& h5_path, bool use_rc, bool codon_phased,
                   const std::string& version) {
   quiet();
-  hid_t fid = H5Fcreate(h5_path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+  H5Handle fapl = write_access();
+  hid_t fid = H5Fcreate(h5_path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, fapl);
   if (fid < 0) throw HDF5Error("Unable to create file " + h5_path);
   H5Handle f(fid, H5Fclose);
   H5Handle g = create_group(f, "sketches");
