@@ -240,51 +240,31 @@ void construct_database(const std::string& db_name, const std::vector<std::strin
   }
 }
 
-std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::string>& names,
-                                 const std::vector<std::vector<std::string>>& files,
-                                 const std::vector<size_t>& kmer_lengths, size_t sketchsize64,
-                                 bool codon_phased, bool use_rc, size_t min_count, bool exact,
-                                 size_t num_threads, int device_id) {
-  std::cerr << "Sketching " << names.size() << " genomes on GPU device " << device_id << std::endl;
-  std::vector<SampleSeq> seqs(names.size());
-  std::vector<std::string> errors(names.size());
-#pragma omp parallel for schedule(dynamic, 1) num_threads(std::max<size_t>(num_threads, 1))
-  for (long i = 0; i < (long)names.size(); ++i) {
-    try {
-      for (const auto& f : files[i]) read_file(f, kmer_lengths.back(), seqs[i]);
-      if (seqs[i].contigs.empty()) errors[i] = names[i] + " contains no sequence";
-    } catch (const std::exception& e) {
-      errors[i] = e.what();
-    }
-  }
-  // as create_sketches (src/api.cpp:100-112): messages on stdout, then throw
-  bool any_error = false;
-  for (const auto& e : errors) {
-    if (!e.empty()) {
-      std::cout << e << std::endl;
-      any_error = true;
-    }
-  }
-  if (any_error) throw std::runtime_error("Errors during sketching");
+namespace {
 
-  std::vector<Sketch> all;
-  all.reserve(names.size());
-  std::vector<int32_t> kl(kmer_lengths.begin(), kmer_lengths.end());
+// Sketch words of samples [s0, s1) from one ppsk_sketch_ex call.
+struct SketchedBatch {
+  size_t s0 = 0, s1 = 0;
+  std::vector<uint64_t> raw;  // [n][K][words]
+  std::vector<double> mh;     // mean bin-0 minhash (reads)
+  std::vector<int32_t> dn;    // densified
+};
+
+// Samples [c0, c1) (already read) on the device, in batches of one kind
+// (count-filtered or not) bounded by their bases.  The contigs of each
+// sample are released once copied into the batch buffer.
+int sketch_range(std::vector<SampleSeq>& seqs, size_t c0, size_t c1,
+                 const std::vector<int32_t>& kl, size_t sketchsize64, bool use_rc,
+                 bool codon_phased, int32_t mc, bool exact, size_t num_threads, int device_id,
+                 std::vector<SketchedBatch>& out, std::string& err) {
   const size_t K = kl.size(), words = sketchsize64 * 14;
-  if (codon_phased) std::cerr << "NB: codon phased seeds are ON" << std::endl;
-  write_new_db(h5, use_rc, codon_phased, SKETCH_VERSION);
-  // reads are counted before binning when min_count > 0 (sketch.cpp:128-149;
-  // create_sketches truncates min_count to uint8, api.cpp:66-69)
-  const int32_t mc = (int32_t)std::min<size_t>(min_count, 255);
   auto filtered = [&](size_t s) { return seqs[s].reads && mc > 0; };
-  // batches of samples of one kind (count-filtered or not), bounded by their
-  // bases (device copy of the sequence)
   const size_t batch_bases = (size_t)1 << 31;
-  size_t s0 = 0;
-  while (s0 < seqs.size()) {
+  size_t s0 = c0;
+  while (s0 < c1) {
     size_t s1 = s0, bases = 0;
-    while (s1 < seqs.size() && (s1 == s0 || (bases + seqs[s1].total <= batch_bases &&
-                                             filtered(s1) == filtered(s0)))) {
+    while (s1 < c1 && (s1 == s0 || (bases + seqs[s1].total <= batch_bases &&
+                                    filtered(s1) == filtered(s0)))) {
       bases += seqs[s1].total;
       ++s1;
     }
@@ -303,22 +283,84 @@ std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::s
     for (long x = 0; x < (long)(s1 - s0); ++x) {
       size_t ci = first_contig[x];
       for (const auto& c : seqs[s0 + x].contigs) std::memcpy(seq.get() + off[ci++], c.data(), c.size());
+      std::vector<std::string>().swap(seqs[s0 + x].contigs);
     }
+    SketchedBatch b;
+    b.s0 = s0;
+    b.s1 = s1;
     const int64_t n = (int64_t)(s1 - s0);
-    std::vector<uint64_t> raw((size_t)n * K * words);
-    std::vector<double> mh(n);
-    std::vector<int32_t> dn(n);
+    b.raw.resize((size_t)n * K * words);
+    b.mh.resize(n);
+    b.dn.resize(n);
     const int rc = ppsk_sketch_ex(device_id, n, seq.get(), (int64_t)seq_len, off.data(),
                                   (int64_t)off.size() - 1, sc.data(), (int32_t)K, kl.data(),
                                   (int32_t)sketchsize64, use_rc ? 1 : 0, codon_phased ? 1 : 0,
-                                  filtered(s0) ? mc : 0, exact ? 1 : 0, raw.data(), mh.data(),
-                                  dn.data());
-    if (rc != PPSK_OK) throw std::runtime_error(ppsk_last_error());
-    std::vector<Sketch> out;
-    for (int64_t x = 0; x < n; ++x) {
-      const SampleSeq& q = seqs[s0 + x];
+                                  filtered(s0) ? mc : 0, exact ? 1 : 0, b.raw.data(),
+                                  b.mh.data(), b.dn.data());
+    if (rc != PPSK_OK) {
+      err = ppsk_last_error();  // thread-local to this thread
+      return rc;
+    }
+    out.push_back(std::move(b));
+    s0 = s1;
+  }
+  return PPSK_OK;
+}
+
+}  // namespace
+
+// As create_sketches (src/api.cpp:39-138): every sample is read in parallel
+// (errors collected for all of them, then reported and thrown), sketched on
+// the device, and only then the new database is created and written.
+std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::string>& names,
+                                 const std::vector<std::vector<std::string>>& files,
+                                 const std::vector<size_t>& kmer_lengths, size_t sketchsize64,
+                                 bool codon_phased, bool use_rc, size_t min_count, bool exact,
+                                 size_t num_threads, int device_id) {
+  std::cerr << "Sketching " << names.size() << " genomes on GPU device " << device_id << std::endl;
+  const size_t n = names.size();
+  const size_t threads = std::max<size_t>(num_threads, 1);
+  std::vector<SampleSeq> seqs(n);
+  std::vector<std::string> errors(n);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+  for (long i = 0; i < (long)n; ++i) {
+    try {
+      for (const auto& f : files[i]) read_file(f, kmer_lengths.back(), seqs[i]);
+      if (seqs[i].contigs.empty()) errors[i] = names[i] + " contains no sequence";
+    } catch (const std::exception& e) {
+      errors[i] = e.what();
+    }
+  }
+  // messages on stdout, then throw (api.cpp:100-112)
+  bool any_error = false;
+  for (const auto& e : errors) {
+    if (!e.empty()) {
+      std::cout << e << std::endl;
+      any_error = true;
+    }
+  }
+  if (any_error) throw std::runtime_error("Errors during sketching");
+
+  std::vector<int32_t> kl(kmer_lengths.begin(), kmer_lengths.end());
+  // reads are counted before binning when min_count > 0 (sketch.cpp:128-149;
+  // create_sketches truncates min_count to uint8, api.cpp:66-69)
+  const int32_t mc = (int32_t)std::min<size_t>(min_count, 255);
+  std::vector<SketchedBatch> batches;
+  std::string device_err;
+  if (sketch_range(seqs, 0, n, kl, sketchsize64, use_rc, codon_phased, mc, exact, threads,
+                   device_id, batches, device_err) != PPSK_OK) {
+    throw std::runtime_error(device_err);
+  }
+
+  if (codon_phased) std::cerr << "NB: codon phased seeds are ON" << std::endl;
+  const size_t K = kl.size(), words = sketchsize64 * 14;
+  std::vector<Sketch> all;
+  all.reserve(n);
+  for (const SketchedBatch& b : batches) {
+    for (size_t x = 0; x < b.s1 - b.s0; ++x) {
+      const SampleSeq& q = seqs[b.s0 + x];
       Sketch s;
-      s.name = names[s0 + x];
+      s.name = names[b.s0 + x];
       s.sketchsize64 = sketchsize64;
       s.bbits = 14;
       s.reads = q.reads;
@@ -327,19 +369,18 @@ std::vector<Sketch> sketch_files(const std::string& h5, const std::vector<std::s
       const double tot = (double)std::min(q.total, MAX_COMPOSITION_SAMPLE);
       s.bases = {q.a / tot, q.c / tot, q.g / tot, q.t / tot};
       // reads: size from the mean bin-0 minhash over k (reference.cpp:49-55)
-      s.seq_length = q.reads ? (size_t)((double)K / mh[x]) : q.total;
+      s.seq_length = q.reads ? (size_t)((double)K / b.mh[x]) : q.total;
       s.kmers = kmer_lengths;
       for (size_t kk = 0; kk < K; ++kk) {
-        const uint64_t* src = raw.data() + ((size_t)x * K + kk) * words;
+        const uint64_t* src = b.raw.data() + (x * K + kk) * words;
         s.usigs.emplace_back(src, src + words);
       }
-      if (dn[x]) std::cerr << "NOTE: " << s.name << " required densification" << std::endl;
-      out.push_back(std::move(s));
+      if (b.dn[x]) std::cerr << "NOTE: " << s.name << " required densification" << std::endl;
+      all.push_back(std::move(s));
     }
-    write_sketches(h5, out);
-    for (auto& x : out) all.push_back(std::move(x));
-    s0 = s1;
   }
+  write_new_db(h5, use_rc, codon_phased, SKETCH_VERSION);
+  write_sketches(h5, all);
   return all;
 }
 
