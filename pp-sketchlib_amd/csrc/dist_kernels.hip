@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 
+#include "log_jaccard.h"
 #include "ppsk_internal.h"
 
 namespace ppsk {
@@ -145,36 +146,6 @@ __device__ __forceinline__ double jaccard_d(uint32_t c, double bins, double r) {
   return (diff * 1.0) / (1.0 - r);
 }
 
-// Natural logarithm of a float Jaccard, in double, to about 1e-15 absolute:
-// x = m 2^e with m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s) with
-// s = (m - 1) / (m + 1), |s| <= 0.1716, summed through s^17 (the next term is
-// below 3e-16), plus e ln 2 with ln 2 split into a short head and a tail
-// (fdlibm's split) so that e * head is exact.  A third of the instructions of
-// the library log; zero, subnormal and non-finite values take the library
-// log (ln 0 = -inf as in the reference's fit).
-__device__ __forceinline__ double log_jaccard(float xf) {
-  if (!(xf >= 1.17549435e-38f && xf <= 3.40282347e+38f)) return log((double)xf);
-  int e = 0;
-  double m = frexp((double)xf, &e);  // [0.5, 1)
-  if (m < 0.70710678118654752440) {
-    m *= 2.0;
-    e -= 1;
-  }
-  const double sv = (m - 1.0) / (m + 1.0);
-  const double s2 = sv * sv;
-  double poly = 2.0 / 17.0;
-  poly = fma(poly, s2, 2.0 / 15.0);
-  poly = fma(poly, s2, 2.0 / 13.0);
-  poly = fma(poly, s2, 2.0 / 11.0);
-  poly = fma(poly, s2, 2.0 / 9.0);
-  poly = fma(poly, s2, 2.0 / 7.0);
-  poly = fma(poly, s2, 2.0 / 5.0);
-  poly = fma(poly, s2, 2.0 / 3.0);
-  poly = fma(poly, s2, 2.0);
-  const double lnm = sv * poly;
-  const double ed = (double)e;
-  return fma(ed, 6.93147180369123816490e-01, fma(ed, 1.90821492927058770002e-10, lnm));
-}
 
 // ---------------------------------------------------------------------------
 // Tile shape and LDS staging

@@ -1,0 +1,93 @@
+"""Accuracy of the regression's logarithm (log_jaccard, csrc/log_jaccard.h).
+
+The distance epilogue takes log J of every float Jaccard in double for the
+core/accessory fit (src/dist/linear_regression.cpp:16-60).  Instead of the
+library log it uses a range reduction and an atanh series; this test builds
+the same header for the host with g++ and checks it against a long-double
+log: every float in [2^-12, 2] (the Jaccards above the fit's tolerance
+5 / 9984 and up to 1, with margin), random floats over the whole positive
+range, and the special values, which must match the library log exactly.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "pp-sketchlib_amd", "csrc")
+
+CHECK = r"""
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include "log_jaccard.h"
+
+static double err(float x) {
+  const long double ref = std::log((long double)x);
+  return (double)std::fabs((long double)log_jaccard(x) - ref);
+}
+
+int main() {
+  double dense = 0;
+  float lo = 1.0f / 4096, hi = 2.0f;
+  uint32_t a, b;
+  std::memcpy(&a, &lo, 4);
+  std::memcpy(&b, &hi, 4);
+  for (uint32_t u = a; u <= b; ++u) {
+    float x;
+    std::memcpy(&x, &u, 4);
+    const double e = err(x);
+    if (e > dense) dense = e;
+  }
+  double wide = 0;
+  std::mt19937 g(1);
+  for (int i = 0; i < 4000000; ++i) {
+    const uint32_t u = g() & 0x7F7FFFFFu;  // positive, finite
+    float x;
+    std::memcpy(&x, &u, 4);
+    if (!(x >= 1.17549435e-38f)) continue;
+    const double e = err(x);
+    if (e > wide) wide = e;
+  }
+  const float sub = 1e-40f;
+  const bool special = std::isinf(log_jaccard(0.0f)) && log_jaccard(0.0f) < 0 &&
+                       std::isnan(log_jaccard(-0.5f)) && log_jaccard(1.0f) == 0.0 &&
+                       log_jaccard(sub) == std::log((double)sub) &&
+                       std::isinf(log_jaccard(INFINITY)) && std::isnan(log_jaccard(NAN));
+  std::printf("%.6e %.6e %d\n", dense, wide, special ? 1 : 0);
+  return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def results(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not found")
+    d = tmp_path_factory.mktemp("logj")
+    src = d / "check.cpp"
+    src.write_text(CHECK)
+    exe = d / "check"
+    # the header's __host__ __device__ qualifiers name nothing for g++
+    subprocess.run(["g++", "-O2", "-std=c++17", "-D__host__=", "-D__device__=", "-I", CSRC,
+                    str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    return float(out[0]), float(out[1]), out[2] == "1"
+
+
+def test_every_float_of_the_jaccard_range(results):
+    dense, _, _ = results
+    assert dense <= 2e-15
+
+
+def test_random_positive_floats(results):
+    _, wide, _ = results
+    # |ln x| reaches 88 at the ends of the float range: 1e-14 is a few ulps
+    assert wide <= 1e-14
+
+
+def test_special_values_follow_the_library_log(results):
+    assert results[2]
