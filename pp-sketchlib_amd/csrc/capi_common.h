@@ -73,6 +73,7 @@ struct DeviceCtx {
   size_t pitems_bytes = 0;
   ppsk::DistParams* pparams = nullptr;  // the call's parameters in device memory
   uint32_t* pqueue = nullptr;         // queue counters
+  hipStream_t tail_stream = nullptr;  // the tail's (tile, k) launch beside the persistent one
   std::mutex mu;
 };
 

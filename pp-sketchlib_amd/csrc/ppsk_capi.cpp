@@ -394,10 +394,22 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
   args.queue_next = c->pqueue;
   args.scratch = (uint16_t*)c->pscratch;
   args.max_k = p.n_k;
+  // The tail's launch goes on a second stream, after the persistent launch
+  // has been queued: its (tile, k) items take the CU slots that the
+  // persistent workgroups leave as they run out of tiles, instead of waiting
+  // for the slowest of them (PPSK_TAIL_STREAM=0: the same stream).  The two
+  // launches share nothing but the output rows they write, which are
+  // disjoint; the main stream waits for the tail before the pass's end event.
+  const char* ts_env = std::getenv("PPSK_TAIL_STREAM");
+  const bool tail_beside = most_tail > 0 && !(ts_env && std::atoi(ts_env) == 0);
+  if (tail_beside && !c->tail_stream) {
+    HIP_TRY(hipStreamCreateWithFlags(&c->tail_stream, hipStreamNonBlocking));
+  }
   for (size_t ci = 0; ci < chunks.size(); ++ci) {
     for (int q = 0; q <= ppsk::BATCH_QUEUES; ++q) args.queue_begin[q] = qb[ci][q];
     hipEvent_t e0 = pool_event(c, 2 * ci), e1 = pool_event(c, 2 * ci + 1);
-    if (!e0 || !e1) return fail(PPSK_ERR_HIP, "event creation failed");
+    hipEvent_t et = pool_event(c, 2 * chunks.size() + ci);
+    if (!e0 || !e1 || !et) return fail(PPSK_ERR_HIP, "event creation failed");
     HIP_TRY(ppsk::launch_batch_reset(c->pqueue, nullptr, 0, c->stream));
     HIP_TRY(hipEventRecord(e0, c->stream));
     HIP_TRY(ppsk::launch_batch(args, variant, grid, c->stream));
@@ -407,7 +419,13 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
       pt.tiles = c->tiles + tail_at[ci];
       pt.n_tiles = (int64_t)tail[ci];
       pt.item_base = 0;
-      HIP_TRY(ppsk::launch_dists(pt, variant, (int64_t)tail[ci] * p.n_k, c->stream));
+      hipStream_t ts = tail_beside ? c->tail_stream : c->stream;
+      if (tail_beside) HIP_TRY(hipStreamWaitEvent(ts, e0, 0));
+      HIP_TRY(ppsk::launch_dists(pt, variant, (int64_t)tail[ci] * p.n_k, ts));
+      if (tail_beside) {
+        HIP_TRY(hipEventRecord(et, ts));
+        HIP_TRY(hipStreamWaitEvent(c->stream, et, 0));
+      }
       tm->launches += 1;
     }
     HIP_TRY(hipEventRecord(e1, c->stream));
