@@ -129,11 +129,12 @@ def test_server_against_oracle(data):
     assert set(zip(ti[sel].tolist(), tj[sel].tolist())) <= got
 
 
-@pytest.mark.parametrize("n_slots", [1, 3])
-def test_requests_split_over_steps(data, n_slots):
+@pytest.mark.parametrize("max_rows,n_slots", [(24, 1), (24, 3), (1, 2)])
+def test_requests_split_over_steps(data, max_rows, n_slots):
     # requests larger than the room in a step are served in pieces of at most
-    # max_rows rows, pieces of different requests sharing steps; each result
-    # equals the same request served in one step of its own
+    # max_rows rows (down to one row per step), pieces of different requests
+    # sharing steps; each result equals the same request served in one step of
+    # its own
     ref, q_raw, q_le = data["ref"], data["q_raw"], data["q_le"]
     spans = [(10, 60), (200, 207), (300, 333), (400, 420), (500, 501), (600, 664)]
     with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
@@ -141,17 +142,17 @@ def test_requests_split_over_steps(data, n_slots):
         want = [whole.wait(whole.submit(q_raw[s:e], q_le[s:e])) for s, e in spans]
     assert all(w["n_steps"] == 1 for w in want)
     with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_EQUAL_BASE,
-                  max_rows=24, n_slots=n_slots) as srv:
+                  max_rows=max_rows, n_slots=n_slots) as srv:
         tickets = [srv.submit(q_raw[s:e], q_le[s:e]) for s, e in spans]
         res = [srv.wait(t) for t in tickets]
         st = srv.stats()
     for (s, e), w, r in zip(spans, want, res):
         _same(w, r)
         assert r["n_queries"] == e - s
-        assert r["n_steps"] >= -(-(e - s) // 24)
+        assert r["n_steps"] >= -(-(e - s) // max_rows)
     assert st["rows"] == sum(e - s for s, e in spans)
     assert st["requests"] == len(spans)
-    assert st["steps"] >= -(-st["rows"] // 24)  # no step above max_rows
+    assert st["steps"] >= -(-st["rows"] // max_rows)  # no step above max_rows
     # the 64-query request against the plain path
     _, _, (ki, kj, kd), (ti, tj, td) = _plain(ref, q_raw[600:664], q_le[600:664])
     r = res[-1]
