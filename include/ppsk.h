@@ -293,7 +293,9 @@ int ppsk_server_stats(ppsk_server *server, int64_t *steps, int64_t *rows,
                       int64_t *requests, int32_t *peak_inflight,
                       size_t *peak_arena_bytes, double *gpu_ms);
 
-/* Stops the workers (queued requests fail); no wait may be in progress. */
+/* Stops the workers.  Requests not yet served fail: a ppsk_server_wait in
+ * progress returns PPSK_ERR_STATE, and free returns only after every waiter
+ * has left.  No call on the server may start once free has begun. */
 int ppsk_server_free(ppsk_server *server);
 
 /*

@@ -152,6 +152,7 @@ struct ppsk_server {
   std::deque<ppsk_request*> pending;
   std::map<int64_t, ppsk_request*> tickets;  // submitted, not yet collected
   int64_t next_ticket = 1;
+  int32_t waiters = 0;  // threads inside ppsk_server_wait
   bool stop = false;
   std::vector<std::thread> workers;
   std::chrono::steady_clock::time_point t0;
@@ -315,7 +316,11 @@ int run_step(ppsk_server* s, Worker& w, std::vector<Piece>& batch, int64_t rows,
     o += pc.n;
   }
   STEP_TRY(ppsk::launch_stripe(staging, rows, 0, n_k, ref->s64, (int32_t)L.nbq, stripes, w.st));
-  STEP_TRY(hipMemsetAsync(tile_done, 0, (size_t)L.chunk_tiles * sizeof(uint32_t), w.st));
+  // per-tile completion counters of the per-(tile, k) kernel (the queue
+  // counters of a persistent step are reset by launch_batch_reset)
+  if (!L.persistent) {
+    STEP_TRY(hipMemsetAsync(tile_done, 0, (size_t)L.chunk_tiles * sizeof(uint32_t), w.st));
+  }
   STEP_TRY(hipMemsetAsync(fail, 0, sizeof(unsigned long long), w.st));
   STEP_TRY(hipMemsetAsync(fail + 1, 0xFF, sizeof(unsigned long long), w.st));
   STEP_TRY(hipMemsetAsync(fail_rows, 0, (size_t)rows * sizeof(uint32_t), w.st));
@@ -651,6 +656,9 @@ void worker_main(ppsk_server* s, Worker* w) {
     }
     std::string err;
     const int rc = run_step(s, *w, batch, rows, s->arena.base + off, L, err);
+    // a failed step may have queued kernels that still write into its arena
+    // block: the block is released only once the stream has drained
+    if (rc != PPSK_OK) hipStreamSynchronize(w->st);
     {
       std::lock_guard<std::mutex> lk(s->mu);
       s->arena.release(off, L.total);
@@ -834,8 +842,11 @@ int ppsk_server_wait(ppsk_server* s, int64_t ticket, ppsk_result** out) {
     auto it = s->tickets.find(ticket);
     if (it == s->tickets.end()) return fail(PPSK_ERR_ARG, "unknown or already collected ticket");
     r = it->second;
+    s->waiters += 1;
     s->cv_done.wait(lk, [&] { return r->done; });
-    s->tickets.erase(it);
+    s->tickets.erase(ticket);
+    s->waiters -= 1;
+    if (s->waiters == 0) s->cv_done.notify_all();  // ppsk_server_free may wait for this
   }
   if (r->status != PPSK_OK) {
     const int rc = r->status;
@@ -883,17 +894,23 @@ int ppsk_server_free(ppsk_server* s) {
   s->cv_work.notify_all();
   for (auto& t : s->workers) t.join();
   {
-    std::lock_guard<std::mutex> lk(s->mu);
-    for (ppsk_request* r : s->pending) {
-      r->status = PPSK_ERR_STATE;
-      r->error = "server stopped";
-      r->done = true;
+    std::unique_lock<std::mutex> lk(s->mu);
+    // every request not yet served fails; waiters collect their failure and
+    // leave before any request or the server is deleted
+    for (auto& kv : s->tickets) {
+      ppsk_request* r = kv.second;
+      if (!r->done) {
+        r->status = PPSK_ERR_STATE;
+        r->error = "server stopped";
+        r->done = true;
+      }
     }
     s->pending.clear();
+    s->cv_done.notify_all();
+    s->cv_done.wait(lk, [&] { return s->waiters == 0; });
     for (auto& kv : s->tickets) delete kv.second;
     s->tickets.clear();
   }
-  s->cv_done.notify_all();
   if (s->arena_mem) {
     hipSetDevice(s->device);
     hipFree(s->arena_mem);
