@@ -123,6 +123,33 @@ int ppsk_dense(ppsk_set *ref, ppsk_set *query, int32_t random_mode,
                uint64_t *n_fail, int64_t *first_fail_row);
 
 /*
+ * Size of the fixed device scratch array of ppsk_dense_noalloc (64 MiB; one
+ * copy per GPU the library runs on, held until the process ends).
+ */
+#define PPSK_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/*
+ * ppsk_dense with device output that allocates no device memory.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, modes, output order and results as ppsk_dense
+ * with out_on_device = 1 (out_device: caller-owned device memory of the
+ * output size).  Its working memory (fit counters, the Monte Carlo table, the
+ * tile list and the per-tile match counts of each launch) is a fixed array of
+ * PPSK_NOALLOC_SCRATCH_BYTES in device memory: any sample count is accepted
+ * (the pass is cut into launches that fit), a Monte Carlo table larger than
+ * the array is rejected with PPSK_ERR_ARG.  The device must have been
+ * initialised (ppsk_device_init or any earlier call on it), else
+ * PPSK_ERR_STATE.  Every call on one GPU shares that GPU's array: calls are
+ * serialised by the library within one process; calls that overlap on one GPU
+ * from outside that lock would race.
+ */
+int ppsk_dense_noalloc(ppsk_set *ref, ppsk_set *query, int32_t random_mode,
+                       int32_t n_clusters, const float *mc_table, double tolerance,
+                       int32_t out_kind, void *out_device, uint64_t *n_fail,
+                       int64_t *first_fail_row);
+
+/*
  * Self kNN, as query_db_sparse: for every sample i the kNN smallest of
  * column dist_col (0 core, 1 accessory) over all j (i == j counts as +inf),
  * ascending per row, i_vec/j_vec/dists of n * kNN entries.  With jaccard

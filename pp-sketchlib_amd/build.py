@@ -22,7 +22,7 @@ HDF5_PREFIX = os.environ.get("PPSK_HDF5_PREFIX", "/opt/conda")
 LIB = os.path.join(HERE, "libppsk.so")
 
 HIP_SOURCES = ["dist_kernels.hip", "select_kernels.hip", "sketch_kernels.hip", "ppsk_capi.cpp",
-               "batch.cpp", "serve.cpp", "sketch_capi.cpp"]
+               "batch.cpp", "serve.cpp", "sketch_capi.cpp", "noalloc.cpp"]
 HOST_SOURCES = ["synth.cpp"]
 
 HIP_FLAGS = [

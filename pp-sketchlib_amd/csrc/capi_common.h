@@ -74,11 +74,18 @@ struct DeviceCtx {
   ppsk::DistParams* pparams = nullptr;  // the call's parameters in device memory
   uint32_t* pqueue = nullptr;         // queue counters
   hipStream_t tail_stream = nullptr;  // the tail's (tile, k) launch beside the persistent one
+  void* noalloc_scratch = nullptr;    // this device's copy of the fixed scratch array
   std::mutex mu;
 };
 
 // Context of a device (created on first use) made current.
 int get_ctx(int device, DeviceCtx** out);
+// Context of a device if it exists (made current), else null; never creates.
+DeviceCtx* find_ctx(int device);
+// Device address of the fixed scratch array of ppsk_dense_noalloc on the
+// current device (hipGetSymbolAddress), called once per device at context
+// creation.
+int noalloc_init(void** scratch);
 // Device buffer of at least `need` bytes (reallocated, contents lost).
 int grow(void** ptr, size_t* have, size_t need);
 // Tile completion counters, zero on return.
@@ -123,6 +130,9 @@ int row_prefetch();
 size_t tail_split_tiles(size_t n, int grid);
 // Device copy of a Monte Carlo table (null for the other modes); caller frees.
 int upload_table(DeviceCtx* c, const ppsk_set* s, const RandomIn& rnd, float** dtab);
+// PPSK_OK, or PPSK_ERR_FIT with the failure count and first row.
+int fit_status(unsigned long long nf, unsigned long long ff);
+void set_timing(const Timing& tm);
 
 }  // namespace capi
 }  // namespace ppsk

@@ -962,4 +962,20 @@ hipError_t launch_batch_reset(uint32_t* queue_next, unsigned long long* fail, in
   return hipGetLastError();
 }
 
+__global__ void tile_reset_kernel(uint32_t* tile_done, int64_t n, unsigned long long* fail) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n) tile_done[x] = 0u;
+  if (x == 0 && fail) {
+    fail[0] = 0ull;
+    fail[1] = ~0ull;
+  }
+}
+
+hipError_t launch_tile_reset(uint32_t* tile_done, int64_t n, unsigned long long* fail,
+                             hipStream_t stream) {
+  const int64_t m = n > 1 ? n : 1;
+  tile_reset_kernel<<<(unsigned)((m + 255) / 256), 256, 0, stream>>>(tile_done, n, fail);
+  return hipGetLastError();
+}
+
 }  // namespace ppsk

@@ -51,11 +51,19 @@ int get_ctx(int device, DeviceCtx** out) {
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&c->fail, 2 * sizeof(unsigned long long)));
+    if (int rc = noalloc_init(&c->noalloc_scratch)) return rc;
     g_ctx[device] = c;
   }
   *out = g_ctx[device];
   HIP_TRY(hipSetDevice(device));
   return PPSK_OK;
+}
+
+DeviceCtx* find_ctx(int device) {
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  if (device < 0 || device >= (int)g_ctx.size() || !g_ctx[device]) return nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  return g_ctx[device];
 }
 
 int grow(void** ptr, size_t* have, size_t need) {

@@ -120,5 +120,9 @@ hipError_t batch_grid(int variant, int* grid);
 hipError_t launch_batch(const BatchArgs& a, int variant, int grid, hipStream_t stream);
 hipError_t launch_batch_reset(uint32_t* queue_next, unsigned long long* fail, int n_jobs,
                               hipStream_t stream);
+// Zeroes n tile completion counters and sets fail ([2]: count, first row; may
+// be null) to (0, max), with a kernel.
+hipError_t launch_tile_reset(uint32_t* tile_done, int64_t n, unsigned long long* fail,
+                             hipStream_t stream);
 
 }  // namespace ppsk

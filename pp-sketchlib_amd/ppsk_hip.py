@@ -32,7 +32,7 @@ EXPORTS = [
     "ppsk_last_timing", "ppsk_synth_sketches", "ppsk_set_prefix", "ppsk_batch_create",
     "ppsk_batch_run", "ppsk_batch_tuning", "ppsk_batch_free", "ppsk_server_create",
     "ppsk_server_submit", "ppsk_server_wait", "ppsk_result_free", "ppsk_server_stats",
-    "ppsk_server_free", "ppsk_sketch", "ppsk_sketch_ex",
+    "ppsk_server_free", "ppsk_sketch", "ppsk_sketch_ex", "ppsk_dense_noalloc",
 ]
 
 
@@ -81,6 +81,8 @@ def lib():
     L.ppsk_set_size.restype = _i64
     L.ppsk_dense.argtypes = [_p, _p, _i32, _i32, _p, _c.c_double, _i32, _p, _i32,
                              _c.POINTER(_u64), _c.POINTER(_i64)]
+    L.ppsk_dense_noalloc.argtypes = [_p, _p, _i32, _i32, _p, _c.c_double, _i32, _p,
+                                     _c.POINTER(_u64), _c.POINTER(_i64)]
     L.ppsk_knn.argtypes = [_p, _i32, _i32, _p, _c.c_double, _i32, _i32, _i32, _p, _p, _p,
                            _c.POINTER(_u64)]
     L.ppsk_threshold.argtypes = [_p, _i32, _i32, _p, _c.c_double, _i32, _c.c_float,
@@ -457,6 +459,21 @@ def dense_device(ref, query=None, random_mode=RANDOM_NONE, table=None, tol=None,
                             _ptr(tab), tol, kind, dptr, 1, _c.byref(nf), _c.byref(ff)),
            allow=(ERR_FIT,))
     return dptr, nf.value
+
+
+def dense_noalloc(ref, dptr, query=None, random_mode=RANDOM_NONE, table=None, tol=None,
+                  out="core_acc"):
+    """ppsk_dense_noalloc into the device buffer dptr.  Returns (n_fail,
+    first_fail_row)."""
+    kind = OUT_KINDS[out]
+    mode, ncl, tab = _random_args(random_mode, table)
+    nf = _u64()
+    ff = _i64()
+    tol = default_tolerance(ref.s64) if tol is None else tol
+    _check(lib().ppsk_dense_noalloc(ref.handle, None if query is None else query.handle, mode,
+                                    ncl, _ptr(tab), tol, kind, dptr, _c.byref(nf),
+                                    _c.byref(ff)), allow=(ERR_FIT,))
+    return nf.value, ff.value
 
 
 def knn(ref, kNN, dist_col=0, random_mode=RANDOM_NONE, table=None, tol=None, jaccard=False):
