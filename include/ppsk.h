@@ -32,6 +32,7 @@ extern "C" {
 #define PPSK_ERR_MEMORY 3  /* device memory exhausted                        */
 #define PPSK_ERR_FIT 4     /* k-mer regression had < 2 valid points          */
 #define PPSK_ERR_STATE 5   /* operation not valid in the current state       */
+#define PPSK_ERR_CANCELLED 6 /* stopped by ppsk_cancel; outputs undefined     */
 
 /* Random-match correction modes (src/random/random_match.hpp:25). */
 #define PPSK_RANDOM_NONE 0       /* RandomMC(): r = 0                          */
@@ -58,6 +59,29 @@ const char *ppsk_version(void);
  */
 int ppsk_device_init(int device_id, size_t *mem_free, size_t *mem_total,
                      size_t *shared_size);
+
+/*
+ * Progress and cancellation of the distance calls on a device (ppsk_dense,
+ * ppsk_dense_noalloc, ppsk_knn, ppsk_threshold, ppsk_self_pass,
+ * ppsk_batch_run), as the reference's GPU progress poll and kill flag
+ * (src/gpu/dist.cu:351-370, src/gpu/cuda.cuh:77-87) and its CPU path's
+ * Ctrl-C check (src/api.cpp:217-226, 285-287).  Calls on one device run one
+ * at a time; these three may be called from any thread at any time, also
+ * while such a call runs, and need no HIP device (device_id in [0, 64)).
+ *
+ * ppsk_progress: (tile, k-mer length) work items finished and planned by the
+ * call running on the device (*running = 1), or by the last one; *pending = 1
+ * while a cancellation request stands.  Any output pointer may be NULL.
+ * ppsk_cancel: asks the call running on the device to stop, and every call
+ * that starts on the device to stop at once, until ppsk_cancel_reset.  The
+ * kernels read the request between work items and finish normally (no
+ * trap); the call waits for them and returns PPSK_ERR_CANCELLED, its outputs
+ * undefined.
+ */
+int ppsk_progress(int device_id, uint64_t *done, uint64_t *total,
+                  int32_t *running, int32_t *pending);
+int ppsk_cancel(int device_id);
+int ppsk_cancel_reset(int device_id);
 
 /*
  * Create a resident sketch set of n_samples samples with the given k-mer
@@ -156,7 +180,8 @@ int ppsk_dense_noalloc(ppsk_set *ref, ppsk_set *query, int32_t random_mode,
  * set, the ranked distance is 1 - J at k-mer length index dist_col and the
  * returned values are J again (src/api.cpp:368-371, 423-429).  Selection and
  * tie order reproduce the reference's std::priority_queue scan over j
- * (src/api.cpp:319-432).  Requires 1 <= kNN <= min(n, 2048).
+ * (src/api.cpp:319-432).  Requires 1 <= kNN <= n; a row's heap lives in LDS
+ * up to kNN = 2048 and in device memory above.
  * Replaces sparseDists (src/gpu/dist.cu:495-791, src/gpu/gpu.hpp:63-73).
  */
 int ppsk_knn(ppsk_set *ref, int32_t random_mode, int32_t n_clusters,
@@ -242,19 +267,27 @@ int ppsk_batch_create(int device_id, const ppsk_job *jobs, int32_t n_jobs,
 
 /*
  * Run every job: one graph launch (queue reset + persistent kernel), replayed
- * on later runs.  variant: tile shape (2: 64 x 256, 4: 32 x 256) or 0 for the
- * autotuner, which on the first run of a batch with an unseen job-shape
- * signature times each shape on the batch itself and keeps the fastest for
- * the process.  n_fail (may be NULL): per-job count of pairs whose regression
- * failed (PPSK_ERR_FIT is returned after all output is written).
- * used_variant / kernel_ms (may be NULL): shape used and its launch time.
+ * on later runs.  variant: the plan -- every job in one tile shape (2:
+ * 64 x 256, 4: 32 x 256, 5: 16 x 256), 6: each job in its own shape (one
+ * persistent launch running several shapes), or 0 for the autotuner.  On the
+ * first run of a batch with an unseen job-shape signature the autotuner
+ * times plans 2, 4 and 5 on the batch itself, derives each shape's
+ * throughput on padded work, gives every job the shape of least estimated
+ * time, times that plan 6 too and keeps the fastest plan for the process.
+ * Before tuning, plan 6 gives each job the shape with the least padding.
+ * n_fail (may be NULL): per-job count of pairs whose regression failed
+ * (PPSK_ERR_FIT is returned after all output is written).  used_variant /
+ * kernel_ms (may be NULL): plan used and its launch time.
  */
 int ppsk_batch_run(ppsk_batch *batch, int32_t variant, uint64_t *n_fail,
                    int32_t *used_variant, double *kernel_ms);
 
-/* Autotuner result: chosen shape and per-shape time (index = shape, 0..5). */
+/* Autotuner result: chosen plan and per-plan time (index = plan, 0..6;
+ * 0 where not timed); shape_per_job (may be NULL, n_jobs entries): each
+ * job's tile shape in plan 6 (after tuning: the shape of least estimated
+ * time; plan 6 is not timed when that is one shape for every job). */
 int ppsk_batch_tuning(const ppsk_batch *batch, int32_t *variant,
-                      double *ms_per_variant);
+                      double *ms_per_variant, int32_t *shape_per_job);
 
 int ppsk_batch_free(ppsk_batch *batch);
 
@@ -278,8 +311,14 @@ int ppsk_batch_free(ppsk_batch *batch);
  * a step's duration and with it the latency a step adds to the requests
  * queued behind it.  A request's results do not depend on the steps it
  * joins.
- * random_mode: PPSK_RANDOM_NONE or PPSK_RANDOM_EQUAL_BASE (query lengths
- * passed at submit).  The reference set must outlive the server.
+ * random_mode: PPSK_RANDOM_NONE, PPSK_RANDOM_EQUAL_BASE (query lengths passed
+ * at submit) or PPSK_RANDOM_MC: the [n_k][C][C] table of ppsk_dense (zeros
+ * where k >= max_k), indexed [cluster of the reference][cluster of the query]
+ * as the reference's query path does (random_match(r1, q_cluster, q_len, k),
+ * src/random/random_match.cpp:185-197, src/api.cpp:237-268), with the
+ * references' clusters from ppsk_set_meta and each query's cluster (its
+ * nearest centroid, RandomMC::closest_cluster, random_match.cpp:230-237)
+ * passed at submit.  The reference set must outlive the server.
  */
 typedef struct ppsk_server ppsk_server;
 
@@ -299,17 +338,19 @@ typedef struct ppsk_result {
   void *owner;       /* library-internal */
 } ppsk_result;
 
-int ppsk_server_create(ppsk_set *ref, int32_t random_mode, double tolerance,
-                       int32_t kNN, int32_t knn_col, float cutoff,
-                       int32_t thr_col, int32_t max_rows, int32_t n_slots,
-                       size_t arena_bytes, ppsk_server **out);
+int ppsk_server_create(ppsk_set *ref, int32_t random_mode, int32_t n_clusters,
+                       const float *mc_table, double tolerance, int32_t kNN,
+                       int32_t knn_col, float cutoff, int32_t thr_col,
+                       int32_t max_rows, int32_t n_slots, size_t arena_bytes,
+                       ppsk_server **out);
 
 /* Queue a request: raw [n_queries][n_k][s64 * 14] in the reference layout,
- * lengths [n_queries] (equal-base mode).  Copies the inputs; never blocks on
- * the GPU. */
+ * lengths [n_queries] (equal-base mode, else may be NULL), clusters
+ * [n_queries] (Monte Carlo mode, else may be NULL).  Copies the inputs;
+ * never blocks on the GPU. */
 int ppsk_server_submit(ppsk_server *server, int64_t n_queries,
                        const uint64_t *raw, const uint64_t *lengths,
-                       int64_t *ticket);
+                       const uint16_t *clusters, int64_t *ticket);
 
 /* Block until the request is served; release the result with
  * ppsk_result_free.  Each ticket can be collected once. */

@@ -1,5 +1,16 @@
 // Batched query-vs-ref jobs (include/ppsk.h, ppsk_batch_*): plan building,
 // graph capture of the persistent launch, and the tile-shape autotuner.
+//
+// Plans (one captured graph each), by index:
+//   2, 4  every job in 64 x 256 / 32 x 256 tiles, on that shape's own
+//         persistent kernel;
+//   5     every job in 16 x 256 tiles, on the mixed-shape kernel;
+//   6     each job in its own shape (per-job choice below), on the
+//         mixed-shape kernel.
+// The autotuner times plans 2, 4 and 5 on the batch itself, derives each
+// shape's throughput on padded work from those runs, gives every job the
+// shape with the least estimated time (padded work / throughput), times that
+// per-job plan too and keeps the fastest of the four.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,19 +47,69 @@ struct ppsk_batch {
   uint32_t* d_queue = nullptr;          // [BATCH_QUEUES]
   uint16_t* scratch = nullptr;
   size_t scratch_bytes = 0;
-  Plan plans[ppsk::N_VARIANTS + 1];
-  int variant = 0;                      // chosen tile shape (0: not chosen yet)
-  double tune_ms[ppsk::N_VARIANTS + 1] = {};
+  Plan plans[ppsk::BATCH_MIXED + 1];
+  int variant = 0;                      // chosen plan (0: not chosen yet)
+  double tune_ms[ppsk::BATCH_MIXED + 1] = {};
+  std::vector<int> job_shape;           // plan 6: tile shape per job
   uint64_t signature = 0;
 };
 
 namespace {
 
-// tile shapes the persistent kernel is built for, tried by the autotuner
-const int kBatchVariants[] = {2, 4};
+// plans a batch can run (see the top of this file); the first three are the
+// uniform shapes the autotuner times
+const int kBatchPlans[] = {2, 4, 5, ppsk::BATCH_MIXED};
+const int kUniformPlans[] = {2, 4, 5};
 
+struct Tuned {
+  uint64_t signature;
+  int plan;
+  std::vector<int> job_shape;
+};
 std::mutex g_tune_mu;
-std::vector<std::pair<uint64_t, int>> g_tuned;  // job-shape signature -> tile shape
+std::vector<Tuned> g_tuned;  // job-shape signature -> plan (and per-job shapes)
+
+// Kernel that runs a plan.
+int plan_kernel(int plan) { return plan == 2 || plan == 4 ? plan : ppsk::BATCH_MIXED; }
+
+// Tile shape of job j in a plan.
+int plan_shape(const ppsk_batch* b, int plan, int j) {
+  return plan == ppsk::BATCH_MIXED ? b->job_shape[j] : plan;
+}
+
+// Tiles of job j in tile shape v (padding included).
+int64_t job_tiles(const ppsk_job& jb, int v) {
+  const int tr = ppsk::variant_tile_rows(v), tc = ppsk::variant_tile_cols(v);
+  const int64_t nbr = (jb.query->n + tr - 1) / tr, nct = (jb.ref->n + tc - 1) / tc;
+  if (jb.query != jb.ref) return nbr * nct;
+  int64_t t = 0;  // self job: upper-triangle tiles, as make_tiles keeps them
+  for (int64_t ct = 0; ct < nct; ++ct) t += std::min<int64_t>(nbr, ((ct + 1) * tc - 1 + tr - 1) / tr);
+  return t;
+}
+
+// int32 matching operations of job j's padded tiles in shape v.
+double padded_ops(const ppsk_job& jb, int v) {
+  return (double)job_tiles(jb, v) * ppsk::variant_tile_rows(v) * ppsk::variant_tile_cols(v) *
+         jb.ref->n_k * jb.ref->s64 * 30.0;
+}
+
+// Per-job shapes: least padded work / shape throughput (ops per ms).  With
+// no measured throughputs (rate <= 0) all shapes count as equally fast.
+std::vector<int> choose_shapes(const ppsk_batch* b, const double* rate) {
+  std::vector<int> s(b->n_jobs, 2);
+  for (int j = 0; j < b->n_jobs; ++j) {
+    double best = 0.0;
+    for (int v : kUniformPlans) {  // larger tiles first: they win ties
+      const double r = rate[v] > 0 ? rate[v] : 1.0;
+      const double t = padded_ops(b->jobs[j], v) / r;
+      if (v == 2 || t < best) {
+        best = t;
+        s[j] = v;
+      }
+    }
+  }
+  return s;
+}
 
 uint64_t mix64(uint64_t h, uint64_t v) {
   h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
@@ -72,28 +133,31 @@ uint64_t batch_signature(const ppsk_batch* b) {
 // queue (and the XCD that drains it) gets an equal share of every job and
 // walks compact blocks of it.  Jobs enter the queues most expensive tile
 // first, leaving the cheap tiles for the end of the launch.
-int build_plan(ppsk_batch* b, int variant) {
-  auto& pl = b->plans[variant];
+int build_plan(ppsk_batch* b, int plan) {
+  auto& pl = b->plans[plan];
   if (pl.exec) return PPSK_OK;
-  const int tr = ppsk::variant_tile_rows(variant);
-  const int tc = ppsk::variant_tile_cols(variant);
+  const int kernel = plan_kernel(plan);
+  // most expensive tiles first: the cheap ones fill the end of the launch
+  auto tile_cost = [&](int j) {
+    return (int64_t)b->jobs[j].ref->n_k * b->jobs[j].ref->s64 *
+           ppsk::variant_tile_rows(plan_shape(b, plan, j));
+  };
   std::vector<int> order(b->n_jobs);
   for (int j = 0; j < b->n_jobs; ++j) order[j] = j;
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    const int64_t cx = (int64_t)b->jobs[x].ref->n_k * b->jobs[x].ref->s64;
-    const int64_t cy = (int64_t)b->jobs[y].ref->n_k * b->jobs[y].ref->s64;
-    return cx > cy;
-  });
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return tile_cost(x) > tile_cost(y); });
   std::vector<std::vector<ppsk::BatchItem>> q(ppsk::BATCH_QUEUES);
   std::vector<size_t> gs;
   for (int j : order) {
     const ppsk_job& jb = b->jobs[j];
     const bool self = jb.query == jb.ref;
+    const int shape = plan_shape(b, plan, j);
+    const int tr = ppsk::variant_tile_rows(shape), tc = ppsk::variant_tile_cols(shape);
     auto tiles = make_tiles(jb.query->n, jb.ref->n, tr, tc, self, 0, jb.query->n, tile_group(), &gs);
     const size_t n = tiles.size();
     for (int qi = 0; qi < ppsk::BATCH_QUEUES; ++qi) {
       const size_t a = n * qi / ppsk::BATCH_QUEUES, e = n * (qi + 1) / ppsk::BATCH_QUEUES;
-      for (size_t x = a; x < e; ++x) q[qi].push_back({j, tiles[x].bi, tiles[x].ct, 0});
+      for (size_t x = a; x < e; ++x) q[qi].push_back({j, tiles[x].bi, tiles[x].ct, shape});
     }
   }
   std::vector<ppsk::BatchItem> all;
@@ -105,13 +169,15 @@ int build_plan(ppsk_batch* b, int variant) {
   pl.n_items = (int64_t)all.size();
   DeviceCtx* c = nullptr;
   if (int rc = get_ctx(b->device, &c)) return rc;
-  HIP_TRY(ppsk::batch_grid(variant, &pl.grid));
+  HIP_TRY(ppsk::batch_grid(kernel, &pl.grid));
   if (!b->scratch) {
-    // sized once for every shape: captured graphs keep this address
+    // sized once for every kernel: captured graphs keep this address (the
+    // mixed-shape kernel's blocks hold the largest shape's counts)
     size_t need = 0;
-    for (int v : kBatchVariants) {
+    for (int k : {2, 4, ppsk::BATCH_MIXED}) {
       int g = 0;
-      HIP_TRY(ppsk::batch_grid(v, &g));
+      HIP_TRY(ppsk::batch_grid(k, &g));
+      const int v = k == ppsk::BATCH_MIXED ? 2 : k;
       need = std::max(need, (size_t)g * b->max_k * ppsk::variant_tile_rows(v) *
                                 ppsk::variant_tile_cols(v) * sizeof(uint16_t));
     }
@@ -128,11 +194,12 @@ int build_plan(ppsk_batch* b, int variant) {
   a.queue_next = b->d_queue;
   a.scratch = b->scratch;
   a.max_k = b->max_k;
+  a.ctl = c->ctl;
   // both launches captured once, replayed by every run; the counters are
   // reset by a kernel (no memset node)
   HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
   hipError_t e = ppsk::launch_batch_reset(b->d_queue, b->d_fail, b->n_jobs, c->stream);
-  if (e == hipSuccess) e = ppsk::launch_batch(a, variant, pl.grid, c->stream);
+  if (e == hipSuccess) e = ppsk::launch_batch(a, kernel, pl.grid, c->stream);
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(c->stream, &g);
   if (e == hipSuccess) e = ec;
@@ -142,20 +209,27 @@ int build_plan(ppsk_batch* b, int variant) {
   return PPSK_OK;
 }
 
-int run_plan(ppsk_batch* b, int variant, double* ms) {
+int run_plan(ppsk_batch* b, int plan, double* ms) {
   DeviceCtx* c = nullptr;
   if (int rc = get_ctx(b->device, &c)) return rc;
-  if (int rc = build_plan(b, variant)) return rc;
+  if (int rc = build_plan(b, plan)) return rc;
+  if (int rc = cancel_status(b->device)) return rc;
+  uint64_t items = 0;
+  for (int j = 0; j < b->n_jobs; ++j) {
+    items += (uint64_t)job_tiles(b->jobs[j], plan_shape(b, plan, j)) * b->jobs[j].ref->n_k;
+  }
+  CallScope scope(c, items);
   hipEvent_t e0 = pool_event(c, 0), e1 = pool_event(c, 1);
   if (!e0 || !e1) return fail(PPSK_ERR_HIP, "event creation failed");
   HIP_TRY(hipEventRecord(e0, c->stream));
-  HIP_TRY(hipGraphLaunch(b->plans[variant].exec, c->stream));
+  HIP_TRY(hipGraphLaunch(b->plans[plan].exec, c->stream));
   HIP_TRY(hipEventRecord(e1, c->stream));
   HIP_TRY(hipEventSynchronize(e1));
   float f = 0.f;
   HIP_TRY(hipEventElapsedTime(&f, e0, e1));
   *ms = f;
-  return PPSK_OK;
+  // the persistent kernel takes no more items once a cancellation is seen
+  return cancel_status(b->device);
 }
 
 }  // namespace
@@ -231,6 +305,8 @@ int ppsk_batch_create(int device_id, const ppsk_job* jobs, int32_t n_jobs, ppsk_
     return rc;
   }
   b->signature = batch_signature(b);
+  const double none[ppsk::BATCH_MIXED + 1] = {};
+  b->job_shape = choose_shapes(b, none);  // least padding until tuned
   *out = b;
   return PPSK_OK;
 }
@@ -240,9 +316,9 @@ int ppsk_batch_run(ppsk_batch* b, int32_t variant, uint64_t* n_fail, int32_t* us
   if (!b) return fail(PPSK_ERR_ARG, "batch is NULL");
   if (variant != 0) {
     bool ok = false;
-    for (int v : kBatchVariants) ok |= (v == variant);
-    if (!ok) return fail(PPSK_ERR_ARG, "tile shape " + std::to_string(variant) +
-                                           " has no batch kernel (2 or 4, or 0 to autotune)");
+    for (int v : kBatchPlans) ok |= (v == variant);
+    if (!ok) return fail(PPSK_ERR_ARG, "no batch plan " + std::to_string(variant) +
+                                           " (2, 4, 5, 6, or 0 to autotune)");
   }
   DeviceCtx* c = nullptr;
   if (int rc = get_ctx(b->device, &c)) return rc;
@@ -252,30 +328,59 @@ int ppsk_batch_run(ppsk_batch* b, int32_t variant, uint64_t* n_fail, int32_t* us
   if (v == 0 && b->variant == 0) {
     if (const char* env = std::getenv("PPSK_BATCH_VARIANT")) {
       const int ev = std::atoi(env);
-      for (int x : kBatchVariants) if (x == ev) b->variant = ev;
+      for (int x : kBatchPlans) if (x == ev) b->variant = ev;
     }
   }
   if (v == 0 && b->variant == 0) {
     std::lock_guard<std::mutex> tl(g_tune_mu);
-    for (auto& kv : g_tuned) if (kv.first == b->signature) b->variant = kv.second;
-  }
-  if (v == 0 && b->variant == 0) {
-    // autotune: every candidate shape runs the whole batch twice (the first
-    // run pays for graph set-up and cold caches); the fastest second run
-    // wins.  Each run produces the complete, correct output.
-    double best = 0.0;
-    for (int cand : kBatchVariants) {
-      double t0 = 0.0, t1 = 0.0;
-      if (int rc = run_plan(b, cand, &t0)) return rc;
-      if (int rc = run_plan(b, cand, &t1)) return rc;
-      b->tune_ms[cand] = t1;
-      if (b->variant == 0 || t1 < best) {
-        best = t1;
-        b->variant = cand;
+    for (auto& t : g_tuned) {
+      if (t.signature == b->signature) {
+        b->variant = t.plan;
+        if (t.plan == ppsk::BATCH_MIXED && b->plans[t.plan].exec == nullptr) {
+          b->job_shape = t.job_shape;
+        }
       }
     }
+  }
+  if (v == 0 && b->variant == 0) {
+    // autotune: each uniform shape runs the whole batch twice (the first run
+    // pays for graph set-up and cold caches) and its second run gives the
+    // shape's throughput on padded work; the per-job plan built from those
+    // throughputs is timed the same way; the fastest plan wins.  Each run
+    // produces the complete, correct output.
+    double rate[ppsk::BATCH_MIXED + 1] = {};
+    auto time_plan = [&](int plan) -> int {
+      double t0 = 0.0, t1 = 0.0;
+      if (int rc = run_plan(b, plan, &t0)) return rc;
+      if (int rc = run_plan(b, plan, &t1)) return rc;
+      b->tune_ms[plan] = t1;
+      if (b->variant == 0 || t1 < b->tune_ms[b->variant]) b->variant = plan;
+      return PPSK_OK;
+    };
+    for (int cand : kUniformPlans) {
+      if (int rc = time_plan(cand)) return rc;
+      double ops = 0.0;
+      for (const ppsk_job& jb : b->jobs) ops += padded_ops(jb, cand);
+      rate[cand] = b->tune_ms[cand] > 0 ? ops / b->tune_ms[cand] : 0.0;
+    }
+    std::vector<int> shapes = choose_shapes(b, rate);
+    if (b->plans[ppsk::BATCH_MIXED].exec && shapes != b->job_shape) {
+      // a per-job plan captured with other shapes: rebuilt on its next use
+      auto& pl = b->plans[ppsk::BATCH_MIXED];
+      hipGraphExecDestroy(pl.exec);
+      hipGraphDestroy(pl.graph);
+      hipFree(pl.items);
+      pl = ppsk_batch::Plan();
+    }
+    b->job_shape = shapes;
+    // one shape for every job: plan 6 is that uniform plan, already timed
+    const bool uniform = std::all_of(shapes.begin(), shapes.end(),
+                                     [&](int x) { return x == shapes[0]; });
+    if (!uniform) {
+      if (int rc = time_plan(ppsk::BATCH_MIXED)) return rc;
+    }
     std::lock_guard<std::mutex> tl(g_tune_mu);
-    g_tuned.emplace_back(b->signature, b->variant);
+    g_tuned.push_back({b->signature, b->variant, b->job_shape});
     v = b->variant;
     ms = b->tune_ms[v];
   } else {
@@ -299,11 +404,13 @@ int ppsk_batch_run(ppsk_batch* b, int32_t variant, uint64_t* n_fail, int32_t* us
   return PPSK_OK;
 }
 
-int ppsk_batch_tuning(const ppsk_batch* b, int32_t* variant, double* ms_per_variant) {
+int ppsk_batch_tuning(const ppsk_batch* b, int32_t* variant, double* ms_per_variant,
+                      int32_t* shape_per_job) {
   if (!b) return fail(PPSK_ERR_ARG, "batch is NULL");
   if (variant) *variant = b->variant;
+  if (shape_per_job) std::copy(b->job_shape.begin(), b->job_shape.end(), shape_per_job);
   if (ms_per_variant) {
-    for (int v = 0; v <= ppsk::N_VARIANTS; ++v) ms_per_variant[v] = b->tune_ms[v];
+    for (int v = 0; v <= ppsk::BATCH_MIXED; ++v) ms_per_variant[v] = b->tune_ms[v];
   }
   return PPSK_OK;
 }

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -75,8 +76,41 @@ struct DeviceCtx {
   uint32_t* pqueue = nullptr;         // queue counters
   hipStream_t tail_stream = nullptr;  // the tail's (tile, k) launch beside the persistent one
   void* noalloc_scratch = nullptr;    // this device's copy of the fixed scratch array
+  ppsk::Control ctl{};                // progress / cancellation pointers of the calls
   std::mutex mu;
 };
+
+// Progress and cancellation state of one device, kept in host memory so
+// that ppsk_progress / ppsk_cancel work from any thread, with or without a
+// device context.  The kernels see the request through a host-mapped word
+// (cancel_host) and report progress through a device counter mirrored into
+// progress_host; both words are allocated with the device context.
+constexpr int MAX_DEVICES = 64;
+struct CallState {
+  std::atomic<uint32_t> requested{0};
+  std::atomic<int32_t> running{0};
+  std::atomic<uint64_t> total{0};
+  std::atomic<uint64_t> seen{0};  // largest progress value read
+  std::atomic<volatile uint32_t*> cancel_host{nullptr};
+  std::atomic<volatile unsigned long long*> progress_host{nullptr};
+};
+CallState& call_state(int device);
+bool cancel_requested(int device);
+// Marks a call on the device as running with `total` work items and clears
+// its progress (the device counter in stream order on `stream`); the
+// destructor marks it finished.
+// The destructor takes the final count from the device counter (the mirror
+// may hold an earlier value when workgroups' stores landed out of order).
+struct CallScope {
+  DeviceCtx* c;
+  CallState& cs;
+  CallScope(DeviceCtx* c, uint64_t total);
+  ~CallScope();
+};
+// PPSK_ERR_CANCELLED with a message when a cancellation stands, else PPSK_OK.
+int cancel_status(int device);
+// Work items ((tile, k-mer length) pairs) of one pass.
+uint64_t pass_items(int64_t n_rows, int64_t n_cols, int n_k, int variant, bool upper);
 
 // Context of a device (created on first use) made current.
 int get_ctx(int device, DeviceCtx** out);

@@ -140,6 +140,24 @@ __device__ __forceinline__ double random_for(const P& p, int kk, int64_t gi, int
   return 0.0;
 }
 
+// Cancellation request of the call (ppsk_cancel), read from host-mapped
+// memory with a system-scope load so that no cache holds a stale value.
+__device__ __forceinline__ bool cancel_seen(const uint32_t* flag) {
+  return flag != nullptr &&
+         __hip_atomic_load(const_cast<uint32_t*>(flag), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM) != 0u;
+}
+
+// n more (tile, k-mer length) items finished: the device counter, mirrored
+// into the host-mapped progress word with a plain store.
+__device__ __forceinline__ void add_progress(const Control& c, unsigned long long n) {
+  if (c.progress == nullptr) return;
+  const unsigned long long old = atomicAdd(c.progress, n);
+  if (c.progress_host != nullptr) {
+    __hip_atomic_store(c.progress_host, old + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 __device__ __forceinline__ double jaccard_d(uint32_t c, double bins, double r) {
   const double obs = (double)c / bins;
   const double diff = obs > r ? (obs - r) : 0.0;
@@ -741,6 +759,7 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
   __shared__ uint4 sB1[Cfg::STAGE_U4];
   __shared__ uint4 sR[PB ? Cfg::ROW_LDS_U4 : 64];  // row prefetch: pipelined shapes only
   __shared__ int s_last;
+  __shared__ int s_stop;
   // the parameters are the kernel's only argument: scalar loads straight
   // from the kernarg segment (no copy)
   CParams* pp = (CParams*)__builtin_amdgcn_kernarg_segment_ptr();
@@ -754,26 +773,35 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
   tile.ct = __builtin_amdgcn_readfirstlane(tile.ct);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-
-  uint32_t cnt[RQ][RR];
-  tile_match<RQ, RR, PB>(pp, tile, kk, sB0, sB1, sR, wave, lane, cnt);
+  // a cancelled call skips the matching; one read per workgroup, shared, so
+  // that every wave takes the same branch (the matching has barriers)
+  if (threadIdx.x == 0) s_stop = cancel_seen(p.ctl.cancel) ? 1 : 0;
+  __syncthreads();
+  const int stop = __builtin_amdgcn_readfirstlane(s_stop);
 
   // per-tile count scratch: [tile][kk][row][col]
   uint16_t* tile_counts = p.counts + t * p.n_k * (int64_t)(Cfg::TROWS * Cfg::TC);
-  store_counts<RQ, RR>(tile_counts, kk, wave, lane, cnt);
+  if (!stop) {
+    uint32_t cnt[RQ][RR];
+    tile_match<RQ, RR, PB>(pp, tile, kk, sB0, sB1, sR, wave, lane, cnt);
+    store_counts<RQ, RR>(tile_counts, kk, wave, lane, cnt);
+  }
 
-  // completion: the last workgroup of this tile runs the epilogue
+  // completion: the last workgroup of this tile runs the epilogue (unless
+  // the call was cancelled, which then holds for every later read)
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned prev = atomicAdd(&p.tile_done[t], 1u);
-    s_last = (prev == (unsigned)(p.n_k - 1)) ? 1 : 0;
+    const bool last = prev == (unsigned)(p.n_k - 1);
+    s_last = last ? (cancel_seen(p.ctl.cancel) ? 2 : 1) : 0;
+    if (last) p.tile_done[t] = 0;  // ready for the next launch
   }
   __syncthreads();
-  if (!s_last) return;
+  if (s_last != 1) return;
   __threadfence();
-  if (threadIdx.x == 0) p.tile_done[t] = 0;  // ready for the next launch
   tile_epilogue<RQ, RR>(pp, tile_counts, tile, wave, lane);
+  if (threadIdx.x == 0) add_progress(p.ctl, (unsigned long long)p.n_k);
 }
 
 // ---------------------------------------------------------------------------
@@ -786,6 +814,15 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
 // go to the workgroup's own scratch block and are read back by the same
 // lanes in the epilogue, so no cross-workgroup synchronisation is needed.
 // ---------------------------------------------------------------------------
+// LDS of the persistent kernels' matching: one set, sized for the largest
+// tile shape (64 x 256) and shared by every shape a kernel runs, so a kernel
+// that mixes shapes holds it once.  Every batch shape stages the same
+// 28 KiB column stage (4 / RR words of 64 RR columns); the row prefetch
+// area grows with the tile rows.
+__shared__ uint4 g_batch_sB0[TileCfg<8, 4>::STAGE_U4];
+__shared__ uint4 g_batch_sB1[TileCfg<8, 4>::STAGE_U4];
+__shared__ uint4 g_batch_sR[TileCfg<8, 4>::ROW_LDS_U4];
+
 // One (tile, k-mer length) of a batch item: counts into the workgroup's
 // scratch block.  Out of line so that the matching loop is register-allocated
 // on its own, exactly as in dist_tiles, instead of around the persistent
@@ -795,9 +832,12 @@ template <int RQ, int RR, int PB>
 __device__ __noinline__ void batch_tile_k(CParams* pp_in, int bi_in, int ct_in, int kk_in,
                                           uint16_t* scratch_in) {
   using Cfg = TileCfg<RQ, RR>;
-  __shared__ uint4 sB0[Cfg::STAGE_U4];
-  __shared__ uint4 sB1[Cfg::STAGE_U4];
-  __shared__ uint4 sR[Cfg::ROW_LDS_U4];
+  static_assert(Cfg::STAGE_U4 == TileCfg<8, 4>::STAGE_U4 &&
+                    Cfg::ROW_LDS_U4 <= TileCfg<8, 4>::ROW_LDS_U4,
+                "batch shapes share the LDS stage buffers");
+  uint4* sB0 = g_batch_sB0;
+  uint4* sB1 = g_batch_sB1;
+  uint4* sR = g_batch_sR;
   const uint64_t pv = (uint64_t)(uintptr_t)pp_in;
   const uint32_t plo = __builtin_amdgcn_readfirstlane((uint32_t)pv);
   const uint32_t phi = __builtin_amdgcn_readfirstlane((uint32_t)(pv >> 32));
@@ -831,6 +871,7 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_batch(BatchArgs a) {
   for (;;) {
     if (threadIdx.x == 0) {
       int64_t got = -1;
+      if (cancel_seen(a.ctl.cancel)) tried = BATCH_QUEUES;  // cancelled: take no more items
       while (tried < BATCH_QUEUES) {
         const int q = (home + tried) % BATCH_QUEUES;
         const int64_t size = a.queue_begin[q + 1] - a.queue_begin[q];
@@ -858,12 +899,81 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_batch(BatchArgs a) {
     const int n_k = pp->n_k;
     for (int kk = 0; kk < n_k; ++kk) batch_tile_k<RQ, RR, PB>(pp, tile.bi, tile.ct, kk, scratch);
     tile_epilogue<RQ, RR>(pp, scratch, tile, wave, lane);
+    if (threadIdx.x == 0) add_progress(a.ctl, (unsigned long long)n_k);
   }
 }
 
-__global__ void batch_reset_kernel(uint32_t* queue_next, unsigned long long* fail, int n_jobs) {
+// One whole tile of a batch item in shape RQ x RR: every k-mer length into
+// the workgroup's scratch block, then the epilogue, then progress.  Out of
+// line: the mixed-shape kernel's frame then only dispatches (a few of its
+// values are saved around the calls).
+template <int RQ, int RR, int PB>
+__device__ __noinline__ void batch_tile(CParams* pp, Tile tile, uint16_t* scratch, Control ctl,
+                                        int wave, int lane) {
+  const int n_k = pp->n_k;
+  for (int kk = 0; kk < n_k; ++kk) batch_tile_k<RQ, RR, PB>(pp, tile.bi, tile.ct, kk, scratch);
+  tile_epilogue<RQ, RR>(pp, scratch, tile, wave, lane);
+  if (threadIdx.x == 0) add_progress(ctl, (unsigned long long)n_k);
+}
+
+// The persistent kernel of dist_batch with a tile shape per item (item.pad:
+// 2 = 64 x 256, 4 = 32 x 256, 5 = 16 x 256), so that each job of a batch runs
+// in the shape that wastes the least of its work on padding rows.  The
+// shapes share the LDS stage buffers; a workgroup's scratch block holds the
+// counts of the largest shape.
+__global__ __launch_bounds__(WAVES * 64, 4) void dist_batch_mixed(BatchArgs a) {
+  __shared__ int64_t s_item;
+  __shared__ int s_tried;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int home = (int)(blockIdx.x % BATCH_QUEUES);
+  uint16_t* scratch = a.scratch + (int64_t)blockIdx.x * a.max_k *
+                                      (int64_t)(TileCfg<8, 4>::TROWS * TileCfg<8, 4>::TC);
+  int tried = 0;  // queues found drained so far (workgroup-uniform)
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int64_t got = -1;
+      if (cancel_seen(a.ctl.cancel)) tried = BATCH_QUEUES;  // cancelled: take no more items
+      while (tried < BATCH_QUEUES) {
+        const int q = (home + tried) % BATCH_QUEUES;
+        const int64_t size = a.queue_begin[q + 1] - a.queue_begin[q];
+        const unsigned idx = atomicAdd(&a.queue_next[q], 1u);
+        if ((int64_t)idx < size) {
+          got = a.queue_begin[q] + idx;
+          break;
+        }
+        ++tried;
+      }
+      s_item = got;
+      s_tried = tried;
+    }
+    __syncthreads();
+    const int64_t it = s_item;
+    tried = s_tried;
+    __syncthreads();  // s_item is rewritten only after every thread read it
+    if (it < 0) break;
+    const BatchItem bi = a.items[it];
+    const int job = __builtin_amdgcn_readfirstlane(bi.job);
+    const int shape = __builtin_amdgcn_readfirstlane(bi.pad);
+    Tile tile;
+    tile.bi = __builtin_amdgcn_readfirstlane(bi.bi);
+    tile.ct = __builtin_amdgcn_readfirstlane(bi.ct);
+    CParams* pp = (CParams*)(a.jobs + job);
+    if (shape == 5) {
+      batch_tile<2, 4, 2>(pp, tile, scratch, a.ctl, wave, lane);
+    } else if (shape == 4) {
+      batch_tile<4, 4, 2>(pp, tile, scratch, a.ctl, wave, lane);
+    } else {
+      batch_tile<8, 4, 2>(pp, tile, scratch, a.ctl, wave, lane);
+    }
+  }
+}
+
+__global__ void batch_reset_kernel(uint32_t* queue_next, unsigned long long* fail, int n_jobs,
+                                   unsigned long long* progress) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x < BATCH_QUEUES) queue_next[x] = 0u;
+  if (x == 0 && progress) *progress = 0ull;
   if (fail && x < n_jobs) {
     fail[2 * x] = 0ull;
     fail[2 * x + 1] = ~0ull;
@@ -920,7 +1030,8 @@ hipError_t launch_dists(const DistParams& p, int variant, int64_t n_items, hipSt
   return hipGetLastError();
 }
 
-// Batch kernels exist for the pipelined shapes only.
+// Batch kernels exist for the pipelined shapes only; BATCH_MIXED runs the
+// shape each item names.
 template <typename F>
 static hipError_t with_batch_kernel(int variant, F f) {
   switch (variant) {
@@ -928,6 +1039,8 @@ static hipError_t with_batch_kernel(int variant, F f) {
       return f(dist_batch<4, 4, 2>);
     case 2:
       return f(dist_batch<8, 4, 2>);
+    case BATCH_MIXED:
+      return f(dist_batch_mixed);
     default:
       return hipErrorInvalidValue;
   }
@@ -956,9 +1069,9 @@ hipError_t launch_batch(const BatchArgs& a, int variant, int grid, hipStream_t s
 }
 
 hipError_t launch_batch_reset(uint32_t* queue_next, unsigned long long* fail, int n_jobs,
-                              hipStream_t stream) {
+                              hipStream_t stream, unsigned long long* progress) {
   const int n = n_jobs > BATCH_QUEUES ? n_jobs : BATCH_QUEUES;
-  batch_reset_kernel<<<(n + 255) / 256, 256, 0, stream>>>(queue_next, fail, n_jobs);
+  batch_reset_kernel<<<(n + 255) / 256, 256, 0, stream>>>(queue_next, fail, n_jobs, progress);
   return hipGetLastError();
 }
 

@@ -8,10 +8,15 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <chrono>
+#include <cstdio>
 #include <cstring>
+#include <future>
 #include <iostream>
+#include <thread>
 
 #include "api.hpp"
+#include "ppsk.h"
 
 namespace py = pybind11;
 using namespace ppsk_host;
@@ -60,6 +65,57 @@ py::tuple coo_to_py(sparse_coo&& c) {
                         py::cast(std::get<2>(c)));
 }
 
+// Runs fn without the GIL on a worker thread while the calling thread
+// watches for signals, as the reference's distance loops poll
+// PyErr_CheckSignals (src/api.cpp:217-226, 285-287) and its GPU path polls
+// progress and sets a kill flag (src/gpu/dist.cu:351-370): on a pending
+// signal the device's calls are cancelled (ppsk_cancel), the worker returns
+// early, the request is withdrawn (ppsk_cancel_reset) and the Python
+// exception (KeyboardInterrupt for Ctrl-C) is raised here.  A call running
+// for more than a second prints its progress to stderr, as the reference's
+// GPU path does.
+template <typename F>
+auto interruptible(int device_id, F fn) -> decltype(fn()) {
+  using R = decltype(fn());
+  std::packaged_task<R()> task(std::move(fn));
+  std::future<R> fut = task.get_future();
+  bool interrupted = false;
+  {
+    py::gil_scoped_release nogil;
+    std::thread worker(std::move(task));
+    const auto t0 = std::chrono::steady_clock::now();
+    auto shown = t0;
+    bool printed = false;
+    while (fut.wait_for(std::chrono::milliseconds(20)) != std::future_status::ready) {
+      if (!interrupted) {
+        py::gil_scoped_acquire gil;
+        if (PyErr_CheckSignals() != 0) {
+          interrupted = true;
+          ppsk_cancel(device_id);
+        }
+      }
+      const auto now = std::chrono::steady_clock::now();
+      if (!interrupted && now - t0 > std::chrono::seconds(1) &&
+          now - shown > std::chrono::milliseconds(250)) {
+        uint64_t done = 0, total = 0;
+        int32_t running = 0;
+        if (ppsk_progress(device_id, &done, &total, &running, nullptr) == PPSK_OK && running &&
+            total > 0) {
+          std::fprintf(stderr, "%cProgress (GPU): %.1lf%%", 13,
+                       100.0 * (double)std::min(done, total) / (double)total);
+          printed = true;
+          shown = now;
+        }
+      }
+    }
+    worker.join();
+    if (interrupted) ppsk_cancel_reset(device_id);
+    if (printed) std::fprintf(stderr, "%cProgress (GPU): 100.0%%\n", 13);
+  }
+  if (interrupted) throw py::error_already_set();
+  return fut.get();
+}
+
 farray queryDatabase(const std::string& ref_db_name, const std::string& query_db_name,
                      const std::vector<std::string>& ref_names,
                      const std::vector<std::string>& query_names,
@@ -78,11 +134,9 @@ farray queryDatabase(const std::string& ref_db_name, const std::string& query_db
     query = load_or_throw(query_db_name, query_names, kmer_lengths);
   }
   RandomMC random = random_correct ? get_random(ref_db_name, ref[0].use_rc) : RandomMC();
-  DenseMatrix d;
-  {
-    py::gil_scoped_release nogil;
-    d = query_db(ref, query, kmer_lengths, random, jaccard, device_id);
-  }
+  DenseMatrix d = interruptible(device_id, [&] {
+    return query_db(ref, query, kmer_lengths, random, jaccard, device_id);
+  });
   return to_numpy(std::move(d));
 }
 
@@ -97,15 +151,12 @@ py::tuple querySelfSparse(const std::string& ref_db_name, const std::vector<std:
   require_ascending(kmer_lengths);
   std::vector<Sketch> ref = load_or_throw(ref_db_name, ref_names, kmer_lengths);
   RandomMC random = random_correct ? get_random(ref_db_name, ref[0].use_rc) : RandomMC();
-  sparse_coo out;
-  {
-    py::gil_scoped_release nogil;
+  sparse_coo out = interruptible(device_id, [&] {
     if (dist_cutoff > 0) {
-      out = query_db_threshold(ref, kmer_lengths, random, dist_cutoff, dist_col, device_id);
-    } else {
-      out = query_db_sparse(ref, kmer_lengths, random, jaccard, (int)kNN, dist_col, device_id);
+      return query_db_threshold(ref, kmer_lengths, random, dist_cutoff, dist_col, device_id);
     }
-  }
+    return query_db_sparse(ref, kmer_lengths, random, jaccard, (int)kNN, dist_col, device_id);
+  });
   return coo_to_py(std::move(out));
 }
 
@@ -382,6 +433,28 @@ PYBIND11_MODULE(pp_sketchlib, m) {
       },
       "What the sequence reader keeps of one sample's files", py::arg("files"),
       py::arg("kmer_len"));
+  m.def(
+      "_interruptProbe",
+      [](double seconds, int device_id) {
+        // stands in for a long distance call: waits until the device's calls
+        // are cancelled (True) or `seconds` pass (False), under the same
+        // signal watch as queryDatabase / querySelfSparse
+        return interruptible(device_id, [=] {
+          const auto t0 = std::chrono::steady_clock::now();
+          while (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() <
+                 seconds) {
+            int32_t pending = 0;
+            if (ppsk_progress(device_id, nullptr, nullptr, nullptr, &pending) != PPSK_OK) {
+              throw std::runtime_error(ppsk_last_error());
+            }
+            if (pending) return true;
+            std::this_thread::sleep_for(std::chrono::milliseconds(1));
+          }
+          return false;
+        });
+      },
+      "Signal handling of the distance calls, without device work", py::arg("seconds"),
+      py::arg("device_id") = 0);
   m.def("writeRandomMatches", &writeRandomMatches,
         "Store Monte Carlo random-match tables in a database's /random group",
         py::arg("db_name"), py::arg("table_keys"), py::arg("table_values"),

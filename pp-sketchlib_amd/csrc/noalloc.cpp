@@ -76,6 +76,7 @@ int ppsk_dense_noalloc(ppsk_set* ref, ppsk_set* query, int32_t random_mode, int3
                                     " not initialised: call ppsk_device_init first");
   }
   std::lock_guard<std::mutex> lk(c->mu);
+  if (int rc = cancel_status(ref->device)) return rc;
   if (n_fail) *n_fail = 0;
   if (first_fail_row) *first_fail_row = -1;
   const int64_t n_rows_out = self ? ref->n * (ref->n - 1) / 2 : query->n * ref->n;
@@ -90,6 +91,7 @@ int ppsk_dense_noalloc(ppsk_set* ref, ppsk_set* query, int32_t random_mode, int3
                                  : 0;
   const size_t fixed = align_up(table_off + table_bytes);
   const int variant = default_variant();
+  CallScope scope(c, pass_items(rows->n, cols->n, ref->n_k, variant, self));
   const int tr = ppsk::variant_tile_rows(variant);
   const int tc = ppsk::variant_tile_cols(variant);
   const size_t per_tile = (size_t)ref->n_k * tr * tc * sizeof(uint16_t);
@@ -127,6 +129,7 @@ int ppsk_dense_noalloc(ppsk_set* ref, ppsk_set* query, int32_t random_mode, int3
   p.fail_first = d_fail + 1;
   p.counts = (uint16_t*)(base + counts_off);
   p.tile_done = d_done;
+  p.ctl = c->ctl;
   if (table_bytes) {
     HIP_TRY(hipMemcpyAsync(base + table_off, mc_table, table_bytes, hipMemcpyHostToDevice,
                            c->stream));
@@ -154,6 +157,8 @@ int ppsk_dense_noalloc(ppsk_set* ref, ppsk_set* query, int32_t random_mode, int3
   HIP_TRY(hipMemcpyAsync(f, d_fail, sizeof(f), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   set_timing(tm);
+  // the kernels skip their remaining work once a cancellation is seen
+  if (int rc = cancel_status(ref->device)) return rc;
   if (n_fail) *n_fail = f[0];
   if (first_fail_row) *first_fail_row = f[0] ? (int64_t)f[1] : -1;
   return fit_status(f[0], f[1]);

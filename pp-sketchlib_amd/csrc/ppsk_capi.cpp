@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -39,7 +40,7 @@ int fail(int code, const std::string& msg) {
 int get_ctx(int device, DeviceCtx** out) {
   int n = 0;
   HIP_TRY(hipGetDeviceCount(&n));
-  if (device < 0 || device >= n) {
+  if (device < 0 || device >= n || device >= MAX_DEVICES) {
     return fail(PPSK_ERR_ARG, "device_id " + std::to_string(device) + " out of range");
   }
   std::lock_guard<std::mutex> lk(g_ctx_mu);
@@ -52,11 +53,88 @@ int get_ctx(int device, DeviceCtx** out) {
     HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&c->fail, 2 * sizeof(unsigned long long)));
     if (int rc = noalloc_init(&c->noalloc_scratch)) return rc;
+    // progress / cancellation words: a device counter and two host-mapped,
+    // coherent words (device_id < MAX_DEVICES here)
+    HIP_TRY(hipMalloc(&c->ctl.progress, sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(c->ctl.progress, 0, sizeof(unsigned long long)));
+    void* hw = nullptr;
+    HIP_TRY(hipHostMalloc(&hw, 64, hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(hw, 0, 64);
+    void* dw = nullptr;
+    HIP_TRY(hipHostGetDevicePointer(&dw, hw, 0));
+    c->ctl.cancel = (const uint32_t*)dw;
+    c->ctl.progress_host = (unsigned long long*)((char*)dw + 8);
+    CallState& cs = call_state(device);
+    cs.progress_host.store((volatile unsigned long long*)((char*)hw + 8));
+    cs.cancel_host.store((volatile uint32_t*)hw);
+    *(volatile uint32_t*)hw = cs.requested.load();  // a request made before the context existed
     g_ctx[device] = c;
   }
   *out = g_ctx[device];
   HIP_TRY(hipSetDevice(device));
   return PPSK_OK;
+}
+
+CallState g_calls[MAX_DEVICES];
+
+CallState& call_state(int device) { return g_calls[device]; }
+
+bool cancel_requested(int device) {
+  return device >= 0 && device < MAX_DEVICES && g_calls[device].requested.load() != 0;
+}
+
+int cancel_status(int device) {
+  if (!cancel_requested(device)) return PPSK_OK;
+  return fail(PPSK_ERR_CANCELLED, "cancelled by ppsk_cancel");
+}
+
+// Largest progress value read so far (stores of different workgroups may
+// arrive out of order).
+uint64_t read_progress(CallState& cs) {
+  uint64_t v = 0;
+  if (!cs.running.load()) return cs.seen.load();  // final count of the last call
+  if (volatile unsigned long long* p = cs.progress_host.load()) v = *p;
+  uint64_t seen = cs.seen.load();
+  while (v > seen && !cs.seen.compare_exchange_weak(seen, v)) {
+  }
+  return std::max(v, seen);
+}
+
+CallScope::CallScope(DeviceCtx* ctx, uint64_t total) : c(ctx), cs(g_calls[ctx->device]) {
+  cs.total.store(total);
+  cs.seen.store(0);
+  if (volatile unsigned long long* p = cs.progress_host.load()) *p = 0;
+  cs.running.store(1);
+  // the device counter starts from zero in stream order
+  hipMemsetAsync(c->ctl.progress, 0, sizeof(unsigned long long), c->stream);
+}
+
+CallScope::~CallScope() {
+  unsigned long long v = 0;
+  if (hipMemcpyAsync(&v, c->ctl.progress, sizeof(v), hipMemcpyDeviceToHost, c->stream) ==
+          hipSuccess &&
+      hipStreamSynchronize(c->stream) == hipSuccess) {
+    cs.seen.store(v);
+  } else {
+    read_progress(cs);
+  }
+  cs.running.store(0);
+}
+
+uint64_t pass_items(int64_t n_rows, int64_t n_cols, int n_k, int variant, bool upper) {
+  const int tr = ppsk::variant_tile_rows(variant), tc = ppsk::variant_tile_cols(variant);
+  const int64_t nbr = (n_rows + tr - 1) / tr, nct = (n_cols + tc - 1) / tc;
+  uint64_t tiles = 0;
+  for (int64_t ct = 0; ct < nct; ++ct) {
+    if (!upper) {
+      tiles += (uint64_t)nbr;
+      continue;
+    }
+    // make_tiles keeps (bi, ct) when bi * tr < (ct + 1) * tc - 1
+    const int64_t lim = (ct + 1) * tc - 1;
+    tiles += (uint64_t)std::min<int64_t>(nbr, (lim + tr - 1) / tr);
+  }
+  return tiles * (uint64_t)n_k;
 }
 
 DeviceCtx* find_ctx(int device) {
@@ -329,8 +407,10 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
   const std::vector<ppsk::Tile> tiles = make_tiles(p.n_rows, p.n_cols, tr, tc, upper, row_begin,
                                                    row_end, finest ? 1 : tile_group(), &gs);
   if (tiles.empty()) return PPSK_OK;
+  // 16-row tiles run on the mixed-shape kernel (every item in that shape)
+  const int kernel = variant == 5 ? ppsk::BATCH_MIXED : variant;
   int grid = 0;
-  HIP_TRY(ppsk::batch_grid(variant, &grid));
+  HIP_TRY(ppsk::batch_grid(kernel, &grid));
   // launches: runs of whole groups of about tiles / 8 (host output) or one
   std::vector<std::pair<size_t, size_t>> chunks;
   size_t target = host_out ? std::max<size_t>((size_t)grid * 4, tiles.size() / 8) : tiles.size();
@@ -367,12 +447,14 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
     for (int q = 0; q < ppsk::BATCH_QUEUES; ++q) {
       qb[ci][q] = (int64_t)items.size();
       for (size_t x = a + n * q / ppsk::BATCH_QUEUES; x < a + n * (q + 1) / ppsk::BATCH_QUEUES; ++x) {
-        items.push_back({0, tiles[x].bi, tiles[x].ct, 0});
+        items.push_back({0, tiles[x].bi, tiles[x].ct, variant});
       }
     }
     qb[ci][ppsk::BATCH_QUEUES] = (int64_t)items.size();
   }
-  const size_t scratch = (size_t)grid * p.n_k * tr * tc * sizeof(uint16_t);
+  // per-workgroup count blocks (the mixed-shape kernel's hold 64 x 256 counts)
+  const size_t scratch = (size_t)grid * p.n_k * (kernel == ppsk::BATCH_MIXED ? 64 * 256 : tr * tc) *
+                         sizeof(uint16_t);
   if (int rc = grow(&c->pscratch, &c->pscratch_bytes, scratch)) return rc;
   if (int rc = grow(&c->pitems, &c->pitems_bytes, items.size() * sizeof(ppsk::BatchItem))) return rc;
   if (!c->pparams) HIP_TRY(hipMalloc(&c->pparams, sizeof(ppsk::DistParams)));
@@ -402,6 +484,7 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
   args.queue_next = c->pqueue;
   args.scratch = (uint16_t*)c->pscratch;
   args.max_k = p.n_k;
+  args.ctl = p.ctl;
   // The tail's launch goes on a second stream, after the persistent launch
   // has been queued: its (tile, k) items take the CU slots that the
   // persistent workgroups leave as they run out of tiles, instead of waiting
@@ -420,7 +503,7 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
     if (!e0 || !e1 || !et) return fail(PPSK_ERR_HIP, "event creation failed");
     HIP_TRY(ppsk::launch_batch_reset(c->pqueue, nullptr, 0, c->stream));
     HIP_TRY(hipEventRecord(e0, c->stream));
-    HIP_TRY(ppsk::launch_batch(args, variant, grid, c->stream));
+    HIP_TRY(ppsk::launch_batch(args, kernel, grid, c->stream));
     tm->launches += 1;
     if (tail[ci] > 0) {
       ppsk::DistParams pt = p;
@@ -462,7 +545,7 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
 int run_dense(DeviceCtx* c, ppsk::DistParams p, int variant, bool upper, int64_t row_begin,
               int64_t row_end, bool self, size_t elem, void* host_out, Timing* tm) {
   const char* e = std::getenv("PPSK_PERSISTENT");
-  bool persistent = variant == 2 || variant == 4;
+  bool persistent = variant == 2 || variant == 4 || variant == 5;
   if (e) {
     persistent = persistent && std::atoi(e) != 0;
   } else if (persistent) {
@@ -470,7 +553,7 @@ int run_dense(DeviceCtx* c, ppsk::DistParams p, int variant, bool upper, int64_t
     const int64_t rt = (row_end - row_begin + tr - 1) / tr, ct = (p.n_cols + tc - 1) / tc;
     const int64_t tiles = upper ? rt * ct / 2 : rt * ct;
     int grid = 0;
-    if (ppsk::batch_grid(variant, &grid) != hipSuccess) grid = 0;
+    if (ppsk::batch_grid(variant == 5 ? ppsk::BATCH_MIXED : variant, &grid) != hipSuccess) grid = 0;
     persistent = tiles >= 2 * (int64_t)grid;
   }
   if (persistent) {
@@ -572,6 +655,7 @@ int dense_pass(DeviceCtx* c, const ppsk_set* rows, const ppsk_set* cols, bool se
   p.mc_ref_is_col = self ? 0 : 1;  // query mode: random_match(r1 = ref, q)
   p.fail_count = c->fail;
   p.fail_first = c->fail + 1;
+  p.ctl = c->ctl;
   int rc = reset_fail(c);
   if (rc == PPSK_OK) {
     const int variant = default_variant();
@@ -579,6 +663,8 @@ int dense_pass(DeviceCtx* c, const ppsk_set* rows, const ppsk_set* cols, bool se
   }
   if (dtab) hipFree(dtab);
   if (rc) return rc;
+  // the kernels skip their remaining work once a cancellation is seen
+  if ((rc = cancel_status(c->device))) return rc;
   return read_fail(c, nf, ff);
 }
 
@@ -588,18 +674,25 @@ int knn_from_triangle(DeviceCtx* c, const float* tri, int64_t n, int col, int kN
                       int64_t* j_vec, float* dists) {
   int64_t *d_i = nullptr, *d_j = nullptr;
   float* d_d = nullptr;
+  void* heap_ws = nullptr;  // row heaps above the LDS limit
   const size_t m = (size_t)n * kNN;
+  const size_t hw = ppsk::knn_heap_workspace(n, kNN);
   hipError_t e = hipMalloc(&d_i, m * sizeof(int64_t));
   if (e == hipSuccess) e = hipMalloc(&d_j, m * sizeof(int64_t));
   if (e == hipSuccess) e = hipMalloc(&d_d, m * sizeof(float));
-  if (e == hipSuccess) e = ppsk::launch_knn_select_triangle(tri, n, col, kNN, d_i, d_j, d_d, c->stream);
+  if (e == hipSuccess && hw) e = hipMalloc(&heap_ws, hw);
+  if (e == hipSuccess) {
+    e = ppsk::launch_knn_select_triangle(tri, n, col, kNN, d_i, d_j, d_d, c->stream, heap_ws);
+  }
   if (e == hipSuccess) e = hipMemcpyAsync(i_vec, d_i, m * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(j_vec, d_j, m * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(dists, d_d, m * sizeof(float), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) hipStreamSynchronize(c->stream);  // nothing queued may outlive its buffers
   if (d_i) hipFree(d_i);
   if (d_j) hipFree(d_j);
   if (d_d) hipFree(d_d);
+  if (heap_ws) hipFree(heap_ws);
   if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? PPSK_ERR_MEMORY : PPSK_ERR_HIP,
                                    std::string("kNN selection: ") + hipGetErrorString(e));
   return PPSK_OK;
@@ -844,6 +937,8 @@ int ppsk_dense(ppsk_set* ref, ppsk_set* query, int32_t random_mode, int32_t n_cl
   DeviceCtx* c = nullptr;
   if (int rc = get_ctx(ref->device, &c)) return rc;
   std::lock_guard<std::mutex> lk(c->mu);
+  if (int rc = cancel_status(ref->device)) return rc;
+  CallScope scope(c, pass_items(rows->n, cols->n, ref->n_k, default_variant(), self));
 
   const int64_t n_rows_out = self ? ref->n * (ref->n - 1) / 2 : query->n * ref->n;
   const size_t elem = out_kind == PPSK_OUT_CORE_ACC ? 2 * sizeof(float)
@@ -873,9 +968,8 @@ int ppsk_knn(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float
              int64_t* j_vec, float* dists, uint64_t* n_fail) {
   set_timing(Timing());
   if (int rc = check_set(ref, "ref")) return rc;
-  if (kNN < 1 || kNN > ref->n || kNN > ppsk::KNN_MAX) {
-    return fail(PPSK_ERR_ARG, "kNN must be in [1, min(number of samples, " +
-                                  std::to_string(ppsk::KNN_MAX) + ")]");
+  if (kNN < 1 || kNN > ref->n) {
+    return fail(PPSK_ERR_ARG, "kNN must be in [1, number of samples]");
   }
   if ((jaccard && dist_col >= ref->n_k) || (dist_col != 0 && dist_col != 1)) {
     return fail(PPSK_ERR_ARG, "dist_col out of range");
@@ -889,13 +983,16 @@ int ppsk_knn(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float
   DeviceCtx* c = nullptr;
   if (int rc = get_ctx(ref->device, &c)) return rc;
   std::lock_guard<std::mutex> lk(c->mu);
+  if (int rc = cancel_status(ref->device)) return rc;
   const int64_t n = ref->n;
   const int32_t kind = jaccard ? ppsk::OUT_ONE_MINUS_J : PPSK_OUT_CORE_ACC;
   Timing tm;
   unsigned long long nf = 0, ff = 0;
   int rc = PPSK_OK;
   const size_t tri_bytes = (size_t)n * (n - 1) / 2 * sizeof(float2);
-  if (table_symmetric(rnd, ref->n_k) && tri_bytes <= ((size_t)96 << 30)) {
+  const bool symmetric = table_symmetric(rnd, ref->n_k) && tri_bytes <= ((size_t)96 << 30);
+  CallScope scope(c, pass_items(n, n, ref->n_k, default_variant(), symmetric));
+  if (symmetric) {
     // distances are symmetric: compute the upper triangle once and read both
     // (i, j) and (j, i) from it
     if ((rc = grow(&c->out, &c->out_bytes, std::max<size_t>(tri_bytes, 8)))) return rc;
@@ -916,9 +1013,12 @@ int ppsk_knn(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float
     if ((rc = upload_table(c, ref, rnd, &dtab))) return rc;
     int64_t *d_i = nullptr, *d_j = nullptr;
     float* d_d = nullptr;
+    void* heap_ws = nullptr;  // row heaps above the LDS limit
+    const size_t hw = ppsk::knn_heap_workspace(band, kNN);
     hipError_t e = hipMalloc(&d_i, (size_t)band * kNN * sizeof(int64_t));
     if (e == hipSuccess) e = hipMalloc(&d_j, (size_t)band * kNN * sizeof(int64_t));
     if (e == hipSuccess) e = hipMalloc(&d_d, (size_t)band * kNN * sizeof(float));
+    if (e == hipSuccess && hw) e = hipMalloc(&heap_ws, hw);
     if (e != hipSuccess) rc = fail(PPSK_ERR_MEMORY, "kNN output buffers");
     if (rc == PPSK_OK) rc = reset_fail(c);
     for (int64_t r0 = 0; r0 < n && rc == PPSK_OK; r0 += band) {
@@ -935,10 +1035,12 @@ int ppsk_knn(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float
       p.mc_ref_is_col = 0;
       p.fail_count = c->fail;
       p.fail_first = c->fail + 1;
+      p.ctl = c->ctl;
       rc = run_dense(c, p, variant, false, r0, r1, true, sizeof(float2), nullptr, &tm);
+      if (rc == PPSK_OK) rc = cancel_status(c->device);
       if (rc) break;
       e = ppsk::launch_knn_select((const float*)c->out, r0, r1 - r0, n, dist_col, kNN, d_i, d_j,
-                                  d_d, c->stream);
+                                  d_d, c->stream, heap_ws);
       const size_t m = (size_t)(r1 - r0) * kNN;
       if (e == hipSuccess)
         e = hipMemcpyAsync(i_vec + r0 * kNN, d_i, m * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
@@ -949,10 +1051,12 @@ int ppsk_knn(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const float
       if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
       if (e != hipSuccess) rc = fail(PPSK_ERR_HIP, std::string("kNN select: ") + hipGetErrorString(e));
     }
+    hipStreamSynchronize(c->stream);  // nothing queued may outlive its buffers
     if (dtab) hipFree(dtab);
     if (d_i) hipFree(d_i);
     if (d_j) hipFree(d_j);
     if (d_d) hipFree(d_d);
+    if (heap_ws) hipFree(heap_ws);
     if (rc == PPSK_OK) rc = read_fail(c, &nf, &ff);
   }
   set_timing(tm);
@@ -985,7 +1089,9 @@ int ppsk_threshold(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const
   DeviceCtx* c = nullptr;
   if (int rc = get_ctx(ref->device, &c)) return rc;
   std::lock_guard<std::mutex> lk(c->mu);
+  if (int rc = cancel_status(ref->device)) return rc;
   const int64_t n = ref->n;
+  CallScope scope(c, pass_items(n, n, ref->n_k, default_variant(), true));
   const size_t tri_bytes = std::max<size_t>((size_t)n * (n - 1) / 2 * sizeof(float2), 8);
   if (int rc = grow(&c->out, &c->out_bytes, tri_bytes)) return rc;
   Timing tm;
@@ -1013,9 +1119,8 @@ int ppsk_self_pass(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const
   if (ref->n_k < 2) {
     return fail(PPSK_ERR_ARG, "core/accessory distances need at least two k-mer lengths");
   }
-  if (kNN < 0 || kNN > ref->n || kNN > ppsk::KNN_MAX) {
-    return fail(PPSK_ERR_ARG, "kNN must be in [0, min(number of samples, " +
-                                  std::to_string(ppsk::KNN_MAX) + ")]");
+  if (kNN < 0 || kNN > ref->n) {
+    return fail(PPSK_ERR_ARG, "kNN must be in [0, number of samples]");
   }
   if (kNN > 0 && ((knn_col != 0 && knn_col != 1) || !knn_i || !knn_j || !knn_d)) {
     return fail(PPSK_ERR_ARG, "bad kNN output arguments");
@@ -1031,7 +1136,9 @@ int ppsk_self_pass(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const
   DeviceCtx* c = nullptr;
   if (int rc = get_ctx(ref->device, &c)) return rc;
   std::lock_guard<std::mutex> lk(c->mu);
+  if (int rc = cancel_status(ref->device)) return rc;
   const int64_t n = ref->n;
+  CallScope scope(c, pass_items(n, n, ref->n_k, default_variant(), true));
   const size_t tri_bytes = std::max<size_t>((size_t)n * (n - 1) / 2 * sizeof(float2), 8);
   if (int rc = grow(&c->out, &c->out_bytes, tri_bytes)) return rc;
   Timing tm;
@@ -1053,6 +1160,35 @@ int ppsk_self_pass(ppsk_set* ref, int32_t random_mode, int32_t n_clusters, const
                                  thr_d, thr_n);
   }
   return rc;
+}
+
+int ppsk_progress(int device_id, uint64_t* done, uint64_t* total, int32_t* running,
+                  int32_t* pending) {
+  if (device_id < 0 || device_id >= MAX_DEVICES) return fail(PPSK_ERR_ARG, "device_id out of range");
+  CallState& cs = call_state(device_id);
+  if (done) *done = read_progress(cs);
+  if (total) *total = cs.total.load();
+  if (running) *running = cs.running.load();
+  if (pending) *pending = cs.requested.load() ? 1 : 0;
+  return PPSK_OK;
+}
+
+int ppsk_cancel(int device_id) {
+  if (device_id < 0 || device_id >= MAX_DEVICES) return fail(PPSK_ERR_ARG, "device_id out of range");
+  CallState& cs = call_state(device_id);
+  cs.requested.store(1);
+  if (volatile uint32_t* w = cs.cancel_host.load()) *w = 1u;
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+  return PPSK_OK;
+}
+
+int ppsk_cancel_reset(int device_id) {
+  if (device_id < 0 || device_id >= MAX_DEVICES) return fail(PPSK_ERR_ARG, "device_id out of range");
+  CallState& cs = call_state(device_id);
+  cs.requested.store(0);
+  if (volatile uint32_t* w = cs.cancel_host.load()) *w = 0u;
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+  return PPSK_OK;
 }
 
 void ppsk_free_host(void* ptr) { std::free(ptr); }

@@ -30,6 +30,18 @@ constexpr int STRIPE_PAD_WORDS = 8;
 enum OutKind : int32_t { OUT_CORE_ACC = 0, OUT_JACCARD = 1, OUT_COUNTS = 2, OUT_ONE_MINUS_J = 3 };
 enum RandomMode : int32_t { RANDOM_NONE = 0, RANDOM_EQUAL_BASE = 1, RANDOM_MC = 2 };
 
+// Progress and cancellation of a call.  cancel: host-mapped flag written by
+// ppsk_cancel, read by the kernels between work items with system-scope
+// loads; progress: device counter of finished (tile, k) items, each update
+// mirrored with a plain system-scope store into the host-mapped word
+// progress_host (stores of different workgroups may land out of order: the
+// host keeps the largest value it has read).
+struct Control {
+  const uint32_t* cancel;
+  unsigned long long* progress;
+  unsigned long long* progress_host;
+};
+
 // Tile: rows [bi * rows, +rows) x cols [ct * cols, +cols) of the variant's
 // tile shape (variant_tile_rows / variant_tile_cols).
 struct Tile {
@@ -70,6 +82,7 @@ struct DistParams {
   unsigned long long* fail_first;  // smallest failing dense row
   uint32_t* fail_rows;             // optional [n_rows] failures per tile row sample
   int32_t row_prefetch;            // pull each stage's row panel into L2 a stage ahead
+  Control ctl;                     // progress and cancellation (all null: neither)
 };
 
 // Persistent batch kernel (dist_batch): work queues and items.
@@ -79,7 +92,7 @@ struct BatchItem {  // one tile of one job: every k-mer length, then the epilogu
   int32_t job;
   int32_t bi;
   int32_t ct;
-  int32_t pad;
+  int32_t pad;  // tile shape (variant) of the item for the mixed-shape kernel
 };
 
 struct BatchArgs {
@@ -89,6 +102,7 @@ struct BatchArgs {
   uint32_t* queue_next;     // [BATCH_QUEUES] next item of each queue (zeroed before launch)
   uint16_t* scratch;        // [grid][max_k][tile rows][tile cols] per-workgroup counts
   int32_t max_k;
+  Control ctl;              // progress and cancellation of the launch (null: neither)
 };
 
 // Launchers (dist_kernels.hip).  All work goes on `stream`.
@@ -112,6 +126,11 @@ int variant_tile_cols(int variant);
 hipError_t launch_dists(const DistParams& p, int variant, int64_t n_items,
                         hipStream_t stream);
 
+// Persistent launch with a tile shape per item (BatchItem.pad = 2, 4 or 5):
+// the `variant` of batch_grid / launch_batch that selects it.  Its scratch
+// holds [grid][max_k][64][256] counts whatever the shapes.
+constexpr int BATCH_MIXED = 6;
+
 // Persistent batch launch: `grid` workgroups (batch_grid: every workgroup
 // resident at once).  batch_reset zeroes the queue counters and sets the
 // per-job fail counters ([n_jobs][2]: count, first row) to (0, max) with a
@@ -119,7 +138,7 @@ hipError_t launch_dists(const DistParams& p, int variant, int64_t n_items,
 hipError_t batch_grid(int variant, int* grid);
 hipError_t launch_batch(const BatchArgs& a, int variant, int grid, hipStream_t stream);
 hipError_t launch_batch_reset(uint32_t* queue_next, unsigned long long* fail, int n_jobs,
-                              hipStream_t stream);
+                              hipStream_t stream, unsigned long long* progress = nullptr);
 // Zeroes n tile completion counters and sets fail ([2]: count, first row; may
 // be null) to (0, max), with a kernel.
 hipError_t launch_tile_reset(uint32_t* tile_done, int64_t n, unsigned long long* fail,

@@ -7,24 +7,31 @@
 
 namespace ppsk {
 
-constexpr int KNN_MAX = 2048;  // heap kept in LDS
+// A row's heap of kNN + 1 entries lives in LDS up to this kNN; larger heaps
+// live in a global-memory workspace (knn_heap_workspace bytes, passed as
+// heap_ws), which the launchers fill with at most KNN_HEAP_ROWS rows at a
+// time.
+constexpr int KNN_LDS_MAX = 2048;
+constexpr int64_t KNN_HEAP_ROWS = 32768;
+size_t knn_heap_workspace(int64_t n_rows, int kNN);
 
 // band: float2 (core, acc) rows [r0, r0 + n_rows) x n columns, row stride n.
 hipError_t launch_knn_select(const float* band, int64_t r0, int64_t n_rows, int64_t n,
                              int dist_col, int kNN, int64_t* i_vec, int64_t* j_vec,
-                             float* dists, hipStream_t stream);
+                             float* dists, hipStream_t stream, void* heap_ws = nullptr);
 
 // Same selection over all n rows, reading (i, j) and (j, i) from the
 // condensed upper triangle (float2 per pair) of a symmetric distance.
 hipError_t launch_knn_select_triangle(const float* tri, int64_t n, int dist_col, int kNN,
                                       int64_t* i_vec, int64_t* j_vec, float* dists,
-                                      hipStream_t stream);
+                                      hipStream_t stream, void* heap_ws = nullptr);
 
 // Query rows against a separate reference set: band [n_rows][n] float2,
 // nothing excluded; i_vec holds the band row.  segs > 1: each row is cut into
 // segs column segments that list their candidates in parallel before one wave
 // per row replays the heap over the lists (same result), for steps of few
-// rows; the lists live in `workspace` (knn_rect_workspace bytes).
+// rows; the lists live in `workspace` (knn_rect_workspace bytes), which also
+// holds the row heaps when kNN > KNN_LDS_MAX (then segs must be 1).
 hipError_t launch_knn_select_rect(const float* band, int64_t n_rows, int64_t n, int dist_col,
                                   int kNN, int64_t* i_vec, int64_t* j_vec, float* dists,
                                   hipStream_t stream, int segs = 1, void* workspace = nullptr);

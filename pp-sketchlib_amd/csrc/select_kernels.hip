@@ -93,14 +93,18 @@ struct TriangleRows {  // condensed upper triangle of n samples (symmetric dista
 
 // excl: column gi is the row's own sample (+inf, self kNN); otherwise the
 // rows are queries against a separate reference set and nothing is excluded.
-template <typename Rows>
-__global__ __launch_bounds__(64) void knn_select_kernel(Rows rows, int64_t r0, int64_t n,
-                                                        int kNN, bool excl,
+// Workgroup b handles row first_row + b (sample r0 + row); GHEAP: its heap is
+// entries [b (kNN + 1), +kNN + 1) of gheap instead of LDS (kNN > KNN_LDS_MAX).
+template <typename Rows, bool GHEAP>
+__global__ __launch_bounds__(64) void knn_select_kernel(Rows rows, int64_t r0, int64_t first_row,
+                                                        int64_t n, int kNN, bool excl,
                                                         int64_t* __restrict__ iv,
                                                         int64_t* __restrict__ jv,
-                                                        float* __restrict__ dv) {
-  extern __shared__ HeapEnt heap[];
-  const int64_t row = blockIdx.x;
+                                                        float* __restrict__ dv,
+                                                        HeapEnt* __restrict__ gheap) {
+  extern __shared__ HeapEnt lds_heap[];
+  HeapEnt* heap = GHEAP ? gheap + (size_t)blockIdx.x * (kNN + 1) : lds_heap;
+  const int64_t row = first_row + blockIdx.x;
   const int64_t gi = r0 + row;
   const int lane = threadIdx.x;
   int len = 0;        // wave-uniform heap size
@@ -287,35 +291,64 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(const float* __restrict__
 
 int knn_rect_capacity(int kNN) { return 16 * kNN + 64; }
 
+size_t knn_heap_workspace(int64_t n_rows, int kNN) {
+  if (kNN <= KNN_LDS_MAX) return 0;
+  const int64_t rows = n_rows < KNN_HEAP_ROWS ? n_rows : KNN_HEAP_ROWS;
+  return (size_t)rows * (kNN + 1) * sizeof(HeapEnt);
+}
+
 size_t knn_rect_workspace(int64_t n_rows, int segs, int kNN) {
-  if (segs <= 1) return 0;
+  if (segs <= 1) return knn_heap_workspace(n_rows, kNN);
   const size_t lists = (size_t)n_rows * segs;
   return lists * knn_rect_capacity(kNN) * sizeof(HeapEnt) + lists * sizeof(int32_t);
 }
 
+// Rows [0, n_rows) of one selection, in launches of at most KNN_HEAP_ROWS
+// rows when the heaps live in the workspace (each launch reuses it); row
+// outputs at row * kNN.  The push writes slot kNN before the pop trims the
+// heap back to kNN entries, hence kNN + 1 entries per heap.
+template <typename Rows>
+hipError_t knn_rows_launch(Rows rows, int64_t r0, int64_t n_rows, int64_t n, int kNN, bool excl,
+                           int64_t* i_vec, int64_t* j_vec, float* dists, hipStream_t stream,
+                           void* heap_ws) {
+  if (kNN <= KNN_LDS_MAX) {
+    knn_select_kernel<Rows, false><<<(unsigned)n_rows, 64, (kNN + 1) * sizeof(HeapEnt), stream>>>(
+        rows, r0, 0, n, kNN, excl, i_vec, j_vec, dists, nullptr);
+    return hipGetLastError();
+  }
+  if (!heap_ws) return hipErrorInvalidValue;
+  for (int64_t f = 0; f < n_rows; f += KNN_HEAP_ROWS) {
+    const int64_t m = n_rows - f < KNN_HEAP_ROWS ? n_rows - f : KNN_HEAP_ROWS;
+    knn_select_kernel<Rows, true><<<(unsigned)m, 64, 0, stream>>>(
+        rows, r0, f, n, kNN, excl, i_vec, j_vec, dists, (HeapEnt*)heap_ws);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_knn_select(const float* band, int64_t r0, int64_t n_rows, int64_t n,
                              int dist_col, int kNN, int64_t* i_vec, int64_t* j_vec,
-                             float* dists, hipStream_t stream) {
-  if (kNN < 1 || kNN > KNN_MAX || kNN > n) return hipErrorInvalidValue;
+                             float* dists, hipStream_t stream, void* heap_ws) {
+  if (kNN < 1 || kNN > n) return hipErrorInvalidValue;
   if (n_rows <= 0) return hipSuccess;
-  // push writes slot kNN before the pop trims the heap back to kNN entries
-  knn_select_kernel<BandRows><<<(unsigned)n_rows, 64, (kNN + 1) * sizeof(HeapEnt), stream>>>(
-      BandRows{band, r0, n, dist_col}, r0, n, kNN, true, i_vec, j_vec, dists);
-  return hipGetLastError();
+  return knn_rows_launch(BandRows{band, r0, n, dist_col}, r0, n_rows, n, kNN, true, i_vec, j_vec,
+                         dists, stream, heap_ws);
 }
 
 hipError_t launch_knn_select_rect(const float* band, int64_t n_rows, int64_t n, int dist_col,
                                   int kNN, int64_t* i_vec, int64_t* j_vec, float* dists,
                                   hipStream_t stream, int segs, void* workspace) {
-  if (kNN < 1 || kNN > KNN_MAX || kNN > n) return hipErrorInvalidValue;
+  if (kNN < 1 || kNN > n) return hipErrorInvalidValue;
   if (n_rows <= 0) return hipSuccess;
   const size_t heap_bytes = (kNN + 1) * sizeof(HeapEnt);
   if (segs <= 1) {
-    knn_select_kernel<BandRows><<<(unsigned)n_rows, 64, heap_bytes, stream>>>(
-        BandRows{band, 0, n, dist_col}, 0, n, kNN, false, i_vec, j_vec, dists);
-    return hipGetLastError();
+    return knn_rows_launch(BandRows{band, 0, n, dist_col}, 0, n_rows, n, kNN, false, i_vec, j_vec,
+                           dists, stream, workspace);
   }
-  if (!workspace || n_rows * segs > INT32_MAX || segs > n) return hipErrorInvalidValue;
+  if (kNN > KNN_LDS_MAX || !workspace || n_rows * segs > INT32_MAX || segs > n) {
+    return hipErrorInvalidValue;
+  }
   const int cap = knn_rect_capacity(kNN);
   HeapEnt* cand = (HeapEnt*)workspace;
   int32_t* cand_n = (int32_t*)(cand + (size_t)n_rows * segs * cap);
@@ -329,12 +362,11 @@ hipError_t launch_knn_select_rect(const float* band, int64_t n_rows, int64_t n, 
 
 hipError_t launch_knn_select_triangle(const float* tri, int64_t n, int dist_col, int kNN,
                                       int64_t* i_vec, int64_t* j_vec, float* dists,
-                                      hipStream_t stream) {
-  if (kNN < 1 || kNN > KNN_MAX || kNN > n) return hipErrorInvalidValue;
+                                      hipStream_t stream, void* heap_ws) {
+  if (kNN < 1 || kNN > n) return hipErrorInvalidValue;
   if (n <= 0) return hipSuccess;
-  knn_select_kernel<TriangleRows><<<(unsigned)n, 64, (kNN + 1) * sizeof(HeapEnt), stream>>>(
-      TriangleRows{tri, n, dist_col}, 0, n, kNN, true, i_vec, j_vec, dists);
-  return hipGetLastError();
+  return knn_rows_launch(TriangleRows{tri, n, dist_col}, 0, n, n, kNN, true, i_vec, j_vec, dists,
+                         stream, heap_ws);
 }
 
 // ---------------------------------------------------------------------------

@@ -120,6 +120,7 @@ struct ppsk_request {
   int64_t n_q = 0;
   std::vector<uint64_t> raw;  // [n_q][n_k][s64 * 14]
   std::vector<double> jv;     // [n_q][n_k] equal-base terms
+  std::vector<uint16_t> cl;   // [n_q] Monte Carlo cluster ids
   ppsk_result res;            // public view of the vectors below
   // kNN lists are sized at submit; each piece fills its own rows in place
   std::vector<int64_t> ki, kj, ti, tj;
@@ -136,6 +137,8 @@ struct ppsk_server {
   ppsk_set* ref = nullptr;
   int device = 0;
   int32_t random_mode = 0;
+  int32_t n_clusters = 0;     // Monte Carlo table [n_k][C][C]
+  float* d_table = nullptr;   // its device copy
   double tolerance = 0;
   int32_t kNN = 0, knn_col = 0, thr_col = 0;
   float cutoff = 0;
@@ -181,7 +184,7 @@ int step_variant(const ppsk_server* s, int64_t rows) {
 // per (tile, k-mer length) over more workgroups (per-tile count scratch,
 // launched in chunks of chunk_tiles).
 struct StepLayout {
-  size_t staging, stripes, jv, band, tiles, scratch, tile_done, fail, fail_rows, params;
+  size_t staging, stripes, jv, clusters, band, tiles, scratch, tile_done, fail, fail_rows, params;
   size_t tail_tiles, tail_counts, tail_done, counts, knn_lists, total;
   int64_t nbq, n_tiles, chunk_tiles, tail;
   int segs;      // column segments per row of the threshold kernels
@@ -211,6 +214,7 @@ StepLayout layout(const ppsk_server* s, int64_t rows) {
   L.staging = take((size_t)rows * s->per_sample);
   L.stripes = take((size_t)r->n_k * L.nbq * r->s64 * ppsk::HL * ppsk::BLK * sizeof(uint32_t));
   L.jv = take((size_t)rows * r->n_k * sizeof(double));
+  L.clusters = take((size_t)rows * sizeof(uint16_t));
   L.band = take((size_t)rows * r->n * 2 * sizeof(float));
   if (L.persistent) {
     L.tail = (int64_t)tail_split_tiles((size_t)L.n_tiles, L.grid);
@@ -295,6 +299,7 @@ int run_step(ppsk_server* s, Worker& w, std::vector<Piece>& batch, int64_t rows,
   uint64_t* staging = (uint64_t*)(mem + L.staging);
   uint32_t* stripes = (uint32_t*)(mem + L.stripes);
   double* jv = (double*)(mem + L.jv);
+  uint16_t* qcl = (uint16_t*)(mem + L.clusters);
   float* band = (float*)(mem + L.band);
   ppsk::Tile* d_tiles = (ppsk::Tile*)(mem + L.tiles);
   uint16_t* scratch = (uint16_t*)(mem + L.scratch);
@@ -312,6 +317,10 @@ int run_step(ppsk_server* s, Worker& w, std::vector<Piece>& batch, int64_t rows,
     if (!r->jv.empty()) {
       STEP_TRY(hipMemcpyAsync(jv + (size_t)o * n_k, r->jv.data() + (size_t)pc.q0 * n_k,
                               (size_t)pc.n * n_k * sizeof(double), hipMemcpyHostToDevice, w.st));
+    }
+    if (!r->cl.empty()) {
+      STEP_TRY(hipMemcpyAsync(qcl + o, r->cl.data() + pc.q0, (size_t)pc.n * sizeof(uint16_t),
+                              hipMemcpyHostToDevice, w.st));
     }
     o += pc.n;
   }
@@ -341,6 +350,13 @@ int run_step(ppsk_server* s, Worker& w, std::vector<Piece>& batch, int64_t rows,
   p.random_mode = s->random_mode;
   p.jv_rows = s->random_mode == PPSK_RANDOM_EQUAL_BASE ? jv : nullptr;
   p.jv_cols = ref->jv;
+  // Monte Carlo: table[k][cluster of the reference][cluster of the query],
+  // the query's cluster being its nearest centroid (random_match(r1 = ref,
+  // q_cluster, ...), src/random/random_match.cpp:185-197, 230-237)
+  p.n_clusters = s->n_clusters;
+  p.mc_table = s->d_table;
+  p.cl_rows = s->random_mode == PPSK_RANDOM_MC ? qcl : nullptr;
+  p.cl_cols = ref->clusters;
   p.mc_ref_is_col = 1;
   p.row_prefetch = row_prefetch();
   p.out_kind = PPSK_OUT_CORE_ACC;
@@ -692,20 +708,33 @@ void worker_main(ppsk_server* s, Worker* w) {
 
 extern "C" {
 
-int ppsk_server_create(ppsk_set* ref, int32_t random_mode, double tolerance, int32_t kNN,
-                       int32_t knn_col, float cutoff, int32_t thr_col, int32_t max_rows,
-                       int32_t n_slots, size_t arena_bytes, ppsk_server** out) {
+int ppsk_server_create(ppsk_set* ref, int32_t random_mode, int32_t n_clusters,
+                       const float* mc_table, double tolerance, int32_t kNN, int32_t knn_col,
+                       float cutoff, int32_t thr_col, int32_t max_rows, int32_t n_slots,
+                       size_t arena_bytes, ppsk_server** out) {
   if (!out) return fail(PPSK_ERR_ARG, "out is NULL");
   *out = nullptr;
   if (int rc = check_set(ref, "ref")) return rc;
   if (ref->n_k < 2) return fail(PPSK_ERR_ARG, "core/accessory distances need at least two k-mer lengths");
-  if (random_mode != PPSK_RANDOM_NONE && random_mode != PPSK_RANDOM_EQUAL_BASE) {
-    return fail(PPSK_ERR_ARG, "serving supports no correction or the equal-base correction");
+  if (random_mode != PPSK_RANDOM_NONE && random_mode != PPSK_RANDOM_EQUAL_BASE &&
+      random_mode != PPSK_RANDOM_MC) {
+    return fail(PPSK_ERR_ARG, "unknown random_mode");
   }
   if (random_mode == PPSK_RANDOM_EQUAL_BASE && !ref->has_lengths) {
     return fail(PPSK_ERR_ARG, "equal-base random correction needs sample lengths (ppsk_set_meta)");
   }
-  if (kNN < 0 || kNN > ppsk::KNN_MAX || kNN > ref->n) return fail(PPSK_ERR_ARG, "kNN out of range");
+  if (random_mode == PPSK_RANDOM_MC) {
+    if (!ref->clusters) {
+      return fail(PPSK_ERR_ARG, "Monte Carlo random correction needs cluster ids (ppsk_set_meta)");
+    }
+    if (n_clusters < 1 || n_clusters > 65536 || !mc_table) {
+      return fail(PPSK_ERR_ARG, "Monte Carlo random correction needs a table");
+    }
+    if (ref->max_cluster >= n_clusters) {
+      return fail(PPSK_ERR_ARG, "cluster id outside the Monte Carlo table");
+    }
+  }
+  if (kNN < 0 || kNN > ref->n) return fail(PPSK_ERR_ARG, "kNN out of range");
   if ((knn_col != 0 && knn_col != 1) || (thr_col != 0 && thr_col != 1)) {
     return fail(PPSK_ERR_ARG, "dist_col out of range");
   }
@@ -720,6 +749,7 @@ int ppsk_server_create(ppsk_set* ref, int32_t random_mode, double tolerance, int
   s->ref = ref;
   s->device = ref->device;
   s->random_mode = random_mode;
+  s->n_clusters = random_mode == PPSK_RANDOM_MC ? n_clusters : 0;
   s->tolerance = tolerance;
   s->kNN = kNN;
   s->knn_col = knn_col;
@@ -746,8 +776,19 @@ int ppsk_server_create(ppsk_set* ref, int32_t random_mode, double tolerance, int
     return fail(PPSK_ERR_MEMORY, "arena smaller than one single-query step (" +
                                      std::to_string(one) + " bytes)");
   }
+  if (random_mode == PPSK_RANDOM_MC) {
+    const size_t tb = (size_t)ref->n_k * n_clusters * n_clusters * sizeof(float);
+    hipError_t et = hipMalloc(&s->d_table, tb);
+    if (et == hipSuccess) et = hipMemcpy(s->d_table, mc_table, tb, hipMemcpyHostToDevice);
+    if (et != hipSuccess) {
+      if (s->d_table) hipFree(s->d_table);
+      delete s;
+      return fail(PPSK_ERR_MEMORY, std::string("serving Monte Carlo table: ") + hipGetErrorString(et));
+    }
+  }
   hipError_t e = hipMalloc(&s->arena_mem, arena_bytes);
   if (e != hipSuccess) {
+    if (s->d_table) hipFree(s->d_table);
     delete s;
     return fail(PPSK_ERR_MEMORY, std::string("serving arena: ") + hipGetErrorString(e));
   }
@@ -782,7 +823,7 @@ int ppsk_server_create(ppsk_set* ref, int32_t random_mode, double tolerance, int
 }
 
 int ppsk_server_submit(ppsk_server* s, int64_t n_queries, const uint64_t* raw,
-                       const uint64_t* lengths, int64_t* ticket) {
+                       const uint64_t* lengths, const uint16_t* clusters, int64_t* ticket) {
   if (!s || !ticket) return fail(PPSK_ERR_ARG, "server / ticket is NULL");
   if (n_queries < 1 || !raw) return fail(PPSK_ERR_ARG, "a request needs at least one query sketch");
   // a request larger than max_rows is served in pieces of at most max_rows
@@ -808,6 +849,19 @@ int ppsk_server_submit(ppsk_server* s, int64_t n_queries, const uint64_t* raw,
         r->jv[(size_t)q * ref->n_k + kk] = 1 - std::pow(base, (double)lengths[q]);
       }
     }
+  }
+  if (s->random_mode == PPSK_RANDOM_MC) {
+    if (!clusters) {
+      delete r;
+      return fail(PPSK_ERR_ARG, "Monte Carlo correction needs query clusters");
+    }
+    for (int64_t q = 0; q < n_queries; ++q) {
+      if (clusters[q] >= s->n_clusters) {
+        delete r;
+        return fail(PPSK_ERR_ARG, "cluster id outside the Monte Carlo table");
+      }
+    }
+    r->cl.assign(clusters, clusters + n_queries);
   }
   if (s->kNN > 0) {
     const size_t nk = (size_t)n_queries * s->kNN;
@@ -911,10 +965,9 @@ int ppsk_server_free(ppsk_server* s) {
     for (auto& kv : s->tickets) delete kv.second;
     s->tickets.clear();
   }
-  if (s->arena_mem) {
-    hipSetDevice(s->device);
-    hipFree(s->arena_mem);
-  }
+  hipSetDevice(s->device);
+  if (s->arena_mem) hipFree(s->arena_mem);
+  if (s->d_table) hipFree(s->d_table);
   delete s;
   return PPSK_OK;
 }

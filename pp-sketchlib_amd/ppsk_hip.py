@@ -13,7 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libppsk.so")
 
-OK, ERR_ARG, ERR_HIP, ERR_MEMORY, ERR_FIT, ERR_STATE = range(6)
+OK, ERR_ARG, ERR_HIP, ERR_MEMORY, ERR_FIT, ERR_STATE, ERR_CANCELLED = range(7)
 RANDOM_NONE, RANDOM_EQUAL_BASE, RANDOM_MC = 0, 1, 2
 OUT_KINDS = {"core_acc": 0, "jaccard": 1, "counts": 2}
 
@@ -33,6 +33,7 @@ EXPORTS = [
     "ppsk_batch_run", "ppsk_batch_tuning", "ppsk_batch_free", "ppsk_server_create",
     "ppsk_server_submit", "ppsk_server_wait", "ppsk_result_free", "ppsk_server_stats",
     "ppsk_server_free", "ppsk_sketch", "ppsk_sketch_ex", "ppsk_dense_noalloc",
+    "ppsk_progress", "ppsk_cancel", "ppsk_cancel_reset",
 ]
 
 
@@ -81,6 +82,10 @@ def lib():
     L.ppsk_set_size.restype = _i64
     L.ppsk_dense.argtypes = [_p, _p, _i32, _i32, _p, _c.c_double, _i32, _p, _i32,
                              _c.POINTER(_u64), _c.POINTER(_i64)]
+    L.ppsk_progress.argtypes = [_c.c_int, _c.POINTER(_u64), _c.POINTER(_u64),
+                                _c.POINTER(_i32), _c.POINTER(_i32)]
+    L.ppsk_cancel.argtypes = [_c.c_int]
+    L.ppsk_cancel_reset.argtypes = [_c.c_int]
     L.ppsk_dense_noalloc.argtypes = [_p, _p, _i32, _i32, _p, _c.c_double, _i32, _p,
                                      _c.POINTER(_u64), _c.POINTER(_i64)]
     L.ppsk_knn.argtypes = [_p, _i32, _i32, _p, _c.c_double, _i32, _i32, _i32, _p, _p, _p,
@@ -104,11 +109,11 @@ def lib():
     L.ppsk_set_prefix.argtypes = [_p, _i64, _c.POINTER(_p)]
     L.ppsk_batch_create.argtypes = [_c.c_int, _c.POINTER(Job), _i32, _c.POINTER(_p)]
     L.ppsk_batch_run.argtypes = [_p, _i32, _p, _c.POINTER(_i32), _c.POINTER(_c.c_double)]
-    L.ppsk_batch_tuning.argtypes = [_p, _c.POINTER(_i32), _p]
+    L.ppsk_batch_tuning.argtypes = [_p, _c.POINTER(_i32), _p, _p]
     L.ppsk_batch_free.argtypes = [_p]
-    L.ppsk_server_create.argtypes = [_p, _i32, _c.c_double, _i32, _i32, _c.c_float, _i32, _i32,
-                                     _i32, _c.c_size_t, _c.POINTER(_p)]
-    L.ppsk_server_submit.argtypes = [_p, _i64, _p, _p, _c.POINTER(_i64)]
+    L.ppsk_server_create.argtypes = [_p, _i32, _i32, _p, _c.c_double, _i32, _i32, _c.c_float,
+                                     _i32, _i32, _i32, _c.c_size_t, _c.POINTER(_p)]
+    L.ppsk_server_submit.argtypes = [_p, _i64, _p, _p, _p, _c.POINTER(_i64)]
     L.ppsk_server_wait.argtypes = [_p, _i64, _c.POINTER(_c.POINTER(Result))]
     L.ppsk_result_free.argtypes = [_c.POINTER(Result)]
     L.ppsk_result_free.restype = None
@@ -142,6 +147,26 @@ def device_init(device=0):
     fr, tot, sh = _c.c_size_t(), _c.c_size_t(), _c.c_size_t()
     _check(lib().ppsk_device_init(device, _c.byref(fr), _c.byref(tot), _c.byref(sh)))
     return fr.value, tot.value, sh.value
+
+
+def progress(device=0):
+    """Progress of the distance call running on a device (or the last one):
+    dict(done, total, running, pending) in (tile, k-mer length) work items;
+    pending is True while a cancellation request stands."""
+    d, t = _u64(), _u64()
+    r, p = _i32(), _i32()
+    _check(lib().ppsk_progress(device, _c.byref(d), _c.byref(t), _c.byref(r), _c.byref(p)))
+    return dict(done=d.value, total=t.value, running=bool(r.value), pending=bool(p.value))
+
+
+def cancel(device=0):
+    """Ask the call running on `device` (and calls starting there until
+    cancel_reset) to stop; they raise PpskError with code ERR_CANCELLED."""
+    _check(lib().ppsk_cancel(device))
+
+
+def cancel_reset(device=0):
+    _check(lib().ppsk_cancel_reset(device))
 
 
 def synth(n, klist, sketchsize64=156, seed=1, dcore_max=0.05, dacc_max=0.30,
@@ -276,10 +301,12 @@ class Batch:
         return nf, used.value, ms.value
 
     def tuning(self):
+        """(chosen plan, time per plan [0..6], per-job tile shape of plan 6)."""
         v = _i32()
-        ms = np.zeros(6, np.float64)
-        _check(lib().ppsk_batch_tuning(self._h, _c.byref(v), _ptr(ms)))
-        return v.value, ms
+        ms = np.zeros(7, np.float64)
+        shapes = np.zeros(len(self.specs), np.int32)
+        _check(lib().ppsk_batch_tuning(self._h, _c.byref(v), _ptr(ms), _ptr(shapes)))
+        return v.value, ms, shapes
 
     def result(self, x):
         dptr, shape, dt = self.specs[x]
@@ -353,20 +380,26 @@ class Server:
     the request's timestamps."""
 
     def __init__(self, ref, kNN=10, knn_col=0, cutoff=0.05, thr_col=0,
-                 random_mode=RANDOM_NONE, tol=None, max_rows=128, n_slots=2, arena_bytes=0):
+                 random_mode=RANDOM_NONE, tol=None, max_rows=128, n_slots=2, arena_bytes=0,
+                 table=None):
         self.ref = ref
         self.random_mode = random_mode
         tol = default_tolerance(ref.s64) if tol is None else tol
+        mode, ncl, tab = _random_args(random_mode, table)
         h = _p()
-        _check(lib().ppsk_server_create(ref.handle, random_mode, tol, kNN, knn_col, cutoff,
-                                        thr_col, max_rows, n_slots, arena_bytes, _c.byref(h)))
+        _check(lib().ppsk_server_create(ref.handle, mode, ncl, _ptr(tab), tol, kNN, knn_col,
+                                        cutoff, thr_col, max_rows, n_slots, arena_bytes,
+                                        _c.byref(h)))
         self._h = h
 
-    def submit(self, raw, lengths=None):
+    def submit(self, raw, lengths=None, clusters=None):
+        """Queue a request; lengths for the equal-base correction, clusters
+        (each query's nearest centroid) for a Monte Carlo table."""
         raw = np.ascontiguousarray(raw, np.uint64)
         le = None if lengths is None else np.ascontiguousarray(lengths, np.uint64)
+        cl = None if clusters is None else np.ascontiguousarray(clusters, np.uint16)
         t = _i64()
-        _check(lib().ppsk_server_submit(self._h, raw.shape[0], _ptr(raw), _ptr(le),
+        _check(lib().ppsk_server_submit(self._h, raw.shape[0], _ptr(raw), _ptr(le), _ptr(cl),
                                         _c.byref(t)))
         return t.value
 

@@ -162,3 +162,46 @@ def test_synthetic_db_round_trip(pp, tmp_path):
     # the writer stores exactly the generator's words; check through h5 attrs
     # by reading back with the module's loader (jaccard of identical ids = 1)
     assert raw.shape == (20, 3, 16 * 14)
+
+
+def _mc_db(tmp_path, pp, below_one_from, stored_names):
+    """A synthetic database whose /random table has mean 1.5 for k below
+    `below_one_from` and 0.01 from there on, with `stored_names` in its
+    cluster table."""
+    db = str(tmp_path / "mcdb")
+    pp.writeSyntheticDatabase(db, 12, [13, 15, 17, 19], sketch_size=1024, seed=5)
+    keys = list(range(3, 32))
+    C = 3
+    tab = np.array([np.full((C, C), 1.5 if k < below_one_from else 0.01, np.float32)
+                    for k in keys])
+    cent = np.full((C, 4), 0.25, np.float32)
+    pp.writeRandomMatches(db, stored_names, [i % C for i in range(len(stored_names))], keys,
+                          tab, cent, 3, 31, True)
+    return db
+
+
+def test_min_supported_k_no_signal_error(tmp_path, pp):
+    # RandomMC::min_supported_k: the first k whose mean random match is below
+    # one (src/random/random_match.cpp:165-182); query_db refuses a smallest
+    # k-mer below it before any distance work (src/api.cpp:148-152)
+    names = [f"sample{i}" for i in range(12)]
+    db = _mc_db(tmp_path, pp, 15, names)
+    with pytest.raises(RuntimeError, match="Smallest k-mer has no signal above random chance"):
+        pp.queryDatabase(db, db, names, names, [13, 15, 17, 19])
+    with pytest.raises(RuntimeError, match="Smallest k-mer has no signal above random chance"):
+        pp.querySelfSparse(db, names, [13, 15, 17, 19], kNN=2)
+
+
+def test_check_present_warning(tmp_path, pp, capfd):
+    # samples missing from the table's cluster list: check_present adds them
+    # (nearest centroid) and query_db warns (src/api.cpp:154-160); the call
+    # then proceeds to the device (here: fails without one, or succeeds)
+    names = [f"sample{i}" for i in range(12)]
+    db = _mc_db(tmp_path, pp, 15, names[:9])
+    try:
+        pp.queryDatabase(db, db, names, names, [15, 17, 19])
+    except RuntimeError as e:
+        assert "no signal" not in str(e)
+    err = capfd.readouterr().err
+    assert "Some members of the reference database were not found in its random match " \
+           "chances. Consider refreshing with addRandom" in err

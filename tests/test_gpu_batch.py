@@ -94,12 +94,30 @@ def batch_run(dbs):
     b.free()
 
 
+def _padded_ops(spec, v):
+    """Matching ops of a job's tiles in shape v (64, 32 or 16 x 256), padding
+    included: query-vs-ref jobs cover every (row tile, column tile)."""
+    ci, nr, nq, _, _ = spec
+    kl, s64 = CLASSES[ci][0], CLASSES[ci][1]
+    tr = {2: 64, 4: 32, 5: 16}[v]
+    tiles = -(-nq // tr) * -(-nr // 256)
+    return tiles * tr * 256 * len(kl) * s64 * 30
+
+
 def test_batch_autotuned_equals_single_calls(batch_run):
     b, views = batch_run
     nf, used, ms = b.run()  # first run: autotuner times every shape
-    assert used in (2, 4) and ms > 0
-    chosen, per = b.tuning()
-    assert chosen == used and per[2] > 0 and per[4] > 0
+    assert used in (2, 4, 5, 6) and ms > 0
+    chosen, per, shapes = b.tuning()
+    assert chosen == used and per[2] > 0 and per[4] > 0 and per[5] > 0
+    # the per-job shapes follow the tuner's model: padded work of the job in a
+    # shape / that shape's measured throughput on the whole batch's padded work
+    ops = {v: [_padded_ops(spec, v) for spec in JOBS] for v in (2, 4, 5)}
+    rate = {v: sum(ops[v]) / per[v] for v in (2, 4, 5)}
+    for x in range(len(JOBS)):
+        est = {v: ops[v][x] / rate[v] for v in (2, 4, 5)}
+        best = min(est.values())
+        assert est[int(shapes[x])] == best, (x, est, shapes[x])
     assert (nf == 0).all()
     for x, spec in enumerate(JOBS):
         d, ref, qry = views[x]
@@ -110,8 +128,10 @@ def test_batch_autotuned_equals_single_calls(batch_run):
         assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), f"job {x} {spec}"
 
 
-@pytest.mark.parametrize("variant", [2, 4])
+@pytest.mark.parametrize("variant", [2, 4, 5, 6])
 def test_batch_each_shape_and_graph_replays(batch_run, variant):
+    # plans 2 / 4: every job in one shape on its own kernel; 5: every job in
+    # 16-row tiles and 6: each job in its own shape, on the mixed-shape kernel
     # three replays of the captured graph, with other device work between
     # them, all equal to the single calls
     b, views = batch_run

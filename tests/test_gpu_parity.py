@@ -97,6 +97,29 @@ def test_knn_matches_heap_order(knn, col):
     assert np.array_equal(jv, rj)
 
 
+@pytest.mark.parametrize("mode", [H.RANDOM_EQUAL_BASE, H.RANDOM_MC])
+def test_knn_above_lds_heap(mode):
+    # kNN = 3,000 > 2,048: each row's heap lives in device memory instead of
+    # LDS; same heap algorithm, so the same lists and tie order as the
+    # reference's priority_queue (src/api.cpp:360-413).  The asymmetric
+    # Monte Carlo table takes the full-square band path.
+    n, k = 3300, 3000
+    raw, le, _ = synth(n)
+    raw, le = with_duplicates(raw, le, [(3, 10), (11, 10), (2000, 10)])
+    table = lt = cl = None
+    if mode == H.RANDOM_MC:
+        table, lt = mc_table(KLIST8, symmetric=False, zero_from=27)
+        cl = (np.arange(n) * 7 % 3).astype(np.uint16)
+    with H.SketchSet(raw, KLIST8, 156, lengths=le, clusters=cl) as s:
+        iv, jv, dv, nf = H.knn(s, k, random_mode=mode, table=table)
+    ri, rj, rd, rnf = O.query_db_sparse(raw, KLIST8, 156, 14, le, k, ref_cluster=cl,
+                                        random=_oracle_random(mode, table, lt))
+    assert nf == rnf == 0
+    assert np.array_equal(iv, ri)
+    assert max_abs_diff(dv, rd) <= CORE_ACC_TOL
+    assert np.array_equal(jv, rj)
+
+
 def test_knn_equals_n():
     raw, le, _ = synth(40)
     with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
@@ -191,10 +214,11 @@ def test_knn_asymmetric_table_band_path():
 @pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 def test_every_tile_shape(monkeypatch, variant, persistent):
     # every compiled tile configuration, self and query mode, against the
-    # oracle (sample counts not a multiple of any tile edge); shapes 2 and 4
-    # run both as the persistent kernel and as one launch per (tile, k)
-    if persistent == "1" and variant not in (2, 4):
-        pytest.skip("the persistent kernel is built for shapes 2 and 4 only")
+    # oracle (sample counts not a multiple of any tile edge); shapes 2, 4 and
+    # 5 run both as a persistent kernel (5: the mixed-shape kernel) and as one
+    # launch per (tile, k)
+    if persistent == "1" and variant not in (2, 4, 5):
+        pytest.skip("the persistent kernels are built for shapes 2, 4 and 5 only")
     monkeypatch.setenv("PPSK_TILE_VARIANT", str(variant))
     monkeypatch.setenv("PPSK_PERSISTENT", persistent)
     raw, le, _ = synth(300)
@@ -223,14 +247,15 @@ def test_row_prefetch_off_is_identical(monkeypatch):
     assert np.array_equal(on.view(np.uint32), off.view(np.uint32))
 
 
-@pytest.mark.parametrize("variant", [2, 4])
+@pytest.mark.parametrize("variant", [2, 4, 5])
 def test_tail_split_persistent_pass(monkeypatch, variant):
     # a pass just over one round of whole tiles: its partial last round runs
     # as (tile, k) items of the per-(tile, k) kernel; outputs are identical to
     # the whole-tile run and to the oracle
     monkeypatch.setenv("PPSK_TILE_VARIANT", str(variant))
     monkeypatch.setenv("PPSK_PERSISTENT", "1")
-    n = 4000 if variant == 2 else 2900
+    # just over one round of the 512 resident workgroups
+    n = {2: 4000, 4: 2900, 5: 2000}[variant]
     raw, le, _ = synth(n)
     with H.SketchSet(raw, KLIST8, 156, lengths=le) as s:
         c, _, _ = H.dense(s, out="counts")

@@ -123,10 +123,54 @@ def test_server_against_oracle(data):
     assert np.abs(r["knn_d"] - kd).max() <= 1e-6
     got_d = acc[r["knn_i"], r["knn_j"]]
     assert np.abs(got_d - r["knn_d"]).max() <= 1e-6
-    ti, tj = np.nonzero(acc < 0.3)
-    sel = np.abs(acc[ti, tj] - 0.3) > 1e-6  # away from the cutoff edge
+    _threshold_matches(r, acc, 0.3)
+
+
+def _threshold_matches(r, dist, cutoff, tol=1e-6):
+    """The server's threshold hits equal the oracle's on both sides, except
+    pairs within the core/accessory tolerance of the cutoff."""
     got = set(zip(r["thr_i"].tolist(), r["thr_j"].tolist()))
-    assert set(zip(ti[sel].tolist(), tj[sel].tolist())) <= got
+    ti, tj = np.nonzero(dist < cutoff)
+    want = set(zip(ti.tolist(), tj.tolist()))
+    assert len(want) > 0
+    for i, j in got ^ want:
+        assert abs(float(dist[i, j]) - cutoff) <= tol, (i, j, float(dist[i, j]))
+    # hits come row-major with their own distances
+    order = np.lexsort((r["thr_j"], r["thr_i"]))
+    assert np.array_equal(order, np.arange(len(order)))
+    assert np.abs(dist[r["thr_i"], r["thr_j"]] - r["thr_d"]).max() <= tol
+
+
+def test_server_monte_carlo_against_oracle(data):
+    # references carry table clusters, each query its nearest centroid's
+    # cluster: table[k][cluster(ref)][cluster(query)], zero from k_max = 27
+    from helpers import mc_table
+    ref_raw, ref_le = data["ref_raw"], data["ref_le"]
+    q_raw, q_le = data["q_raw"], data["q_le"]
+    rcl = (np.arange(N_REF) * 5 % 3).astype(np.uint16)
+    qcl = (np.arange(N_Q) % 3).astype(np.uint16)
+    table, lt = mc_table(KLIST8, symmetric=False, seed=4, zero_from=27)
+    spans = [(5, 6), (100, 129), (400, 470)]
+    with H.SketchSet(ref_raw, KLIST8, S64, lengths=ref_le, clusters=rcl) as ref:
+        with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_MC, table=table,
+                      max_rows=32, n_slots=2) as srv:
+            tickets = [srv.submit(q_raw[s:e], clusters=qcl[s:e]) for s, e in spans]
+            res = [srv.wait(t) for t in tickets]
+        with H.Server(ref, kNN=KNN, cutoff=CUTOFF, random_mode=H.RANDOM_MC, table=table,
+                      max_rows=32, n_slots=1) as bad:
+            with pytest.raises(H.PpskError, match="clusters"):
+                bad.submit(q_raw[:2])
+    for (s, e), r in zip(spans, res):
+        want, _, nf, _ = O.query_db(ref_raw, KLIST8, S64, 14, ref_le, query=q_raw[s:e],
+                                    query_len=q_le[s:e], ref_cluster=rcl,
+                                    query_cluster=qcl[s:e], random=O.Random(2, True, table, lt))
+        assert nf == 0 and r["n_fail"] == 0
+        core = want[:, 0].reshape(e - s, N_REF)
+        ki, kj, kd = O.knn_rows(core, np.full(e - s, -1), KNN)
+        assert np.array_equal(r["knn_i"], ki)
+        assert np.abs(r["knn_d"] - kd).max() <= 1e-6
+        assert np.abs(core[r["knn_i"], r["knn_j"]] - r["knn_d"]).max() <= 1e-6
+        _threshold_matches(r, core, CUTOFF)
 
 
 @pytest.mark.parametrize("max_rows,n_slots", [(24, 1), (24, 3), (1, 2)])
