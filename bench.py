@@ -13,7 +13,14 @@ Prints one JSON line:
              ops per pair = sum_k s64 * (2 * bbits + 2) = 37,440, divided by
              the kernel's HIP-event duration on its launch stream
   cpu_baseline  the CPU oracle (C++/OpenMP restatement of the reference CPU
-             path) on a bounded sample of the same workload, on this host
+             path), compiled on this host with -march=native, on a bounded
+             sample of the same workload: median of 3 runs at 16 threads and
+             of 3 runs on every core of the process's CPU set
+  serving    the metric's latency figure: a resident database of 200,000
+             sketches (same k-mer lengths and sketch size) serving Poisson
+             query batches of 1-512 sketches at 70% of the measured saturation
+             rate with kNN = 10 and threshold 0.05: p50 / p99 request latency
+             and the saturation throughput
 """
 import argparse
 import json
@@ -43,8 +50,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=N_SAMPLES)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-sample", type=int, default=8192,
+    ap.add_argument("--cpu-sample", type=int, default=4096,
                     help="samples of the CPU baseline's all-vs-all sample")
+    ap.add_argument("--cpu-runs", type=int, default=3,
+                    help="CPU baseline runs per thread count (the median is reported)")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving measurement of the headline line")
+    ap.add_argument("--headline-serve-seconds", type=float, default=10.0,
+                    help="Poisson arrival window of the headline line's serving figure")
     ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                     help="run one plain copy kernel of BYTES and exit (HBM counter calibration)")
     ap.add_argument("--workload", default="plain", choices=["plain", "large", "batch", "serve", "sketch", "reads", "genomes"],
@@ -107,24 +120,85 @@ def traffic_from_profile():
     return d.get("hbm_bytes_per_pass")
 
 
-def cpu_baseline(raw, lengths, n_sample, threads):
+def _host_cpu():
+    """CPU model, SMT state and the cores this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = "on" if f.read().strip() == "1" else "off"
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return model, smt, cores
+
+
+def _native_oracle():
+    """Builds the oracle for this host's CPU (-march=native, the reference's
+    own Makefile flag) into a temporary directory and points the oracle
+    module at it; returns the compile flags used."""
+    import subprocess
+    import tempfile
+    out = tempfile.mkdtemp(prefix="ppsk_oracle_")
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native",
+                    f"NATIVE_DIR={out}"], check=True, stdout=subprocess.DEVNULL)
+    os.environ["PPSK_ORACLE_LIB"] = os.path.join(out, "liboracle_native.so")
+    return "-O3 -march=native -ffp-contract=off -fno-fast-math -fopenmp"
+
+
+def cpu_baseline(raw, lengths, n_sample, runs):
+    """Median distances/s of the oracle's all-vs-all over the first n_sample
+    samples: `runs` runs at 16 threads and `runs` on every available core."""
+    import statistics
+    try:
+        flags = _native_oracle()
+    except Exception as e:  # no compiler: the portable build (x86-64-v3)
+        print(f"# native oracle build failed ({e}); using oracle/liboracle.so", file=sys.stderr)
+        flags = "-O3 -march=x86-64-v3 -ffp-contract=off -fno-fast-math -fopenmp"
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     sub = raw[:n_sample]
     le = lengths[:n_sample]
     pairs = n_sample * (n_sample - 1) // 2
-    t0 = time.perf_counter()
-    O.query_db(sub, KLIST, S64, BBITS, le, random=O.Random(1, True), num_threads=threads)
-    dt = time.perf_counter() - t0
+    model, smt, avail = _host_cpu()
+
+    def rate(threads):
+        t0 = time.perf_counter()
+        O.query_db(sub, KLIST, S64, BBITS, le, random=O.Random(1, True), num_threads=threads)
+        return pairs / (time.perf_counter() - t0)
+
+    rate(16)  # first touch: library load, page faults
+    r16 = [rate(16) for _ in range(runs)]
+    rall = [rate(avail) for _ in range(runs)] if avail != 16 else r16
+    med16 = statistics.median(r16)
+    print(f"# cpu baseline: {med16:.3e} distances/s at 16 threads, "
+          f"{statistics.median(rall):.3e} at {avail}", file=sys.stderr, flush=True)
     return {
-        "value": pairs / dt,
+        "value": med16,
         "unit": "distances/s",
-        "cores": threads,
+        "cores": 16,
         "kind": "port",
         "sample": f"all-vs-all of the first {n_sample} of the {N_SAMPLES} samples "
-                  f"({pairs} pairs, {dt:.1f} s), same k-mer lengths / sketch size / "
-                  f"random correction",
+                  f"({pairs} pairs per run), same k-mer lengths / sketch size / random "
+                  f"correction; median of {runs} runs",
+        "runs_16_threads": [round(x, 1) for x in r16],
+        "all_cores": {"threads": avail, "value": round(statistics.median(rall), 1),
+                      "runs": [round(x, 1) for x in rall]},
+        "cpu_model": model,
+        "smt": smt,
+        "build": flags,
     }
 
 
@@ -190,10 +264,17 @@ def main():
     peak_meas = measured_valu_peak()
     if peak_meas:
         roofline["peak_measured"] = round(peak_meas, 3)
+    H.device_free(dout)
+    ref.free()
     cpu = None
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(raw, lengths, min(a.cpu_sample, a.n), threads)
+        cpu = cpu_baseline(raw, lengths, min(a.cpu_sample, a.n), a.cpu_runs)
+    del raw
+    serving = None
+    if not a.no_serve:
+        import bench_workloads as W
+        serving = W.serve_headline(a.serve_refs, a.headline_serve_seconds,
+                                   a.serve_max_rows, a.serve_slots)
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -218,6 +299,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

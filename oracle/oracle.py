@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# PPSK_ORACLE_LIB: another build of the same source (the CPU baseline of
+# bench.py compiles one for its host's CPU)
+_LIB_PATH = os.environ.get("PPSK_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _REF_PATH = os.path.join(_HERE, "_ref", "libref_dist.so")
 
 _c = ctypes

@@ -129,7 +129,7 @@ def batch(a):
     t = time.perf_counter()
     b.run()  # first run: the autotuner times each tile shape on this batch
     tune_s = time.perf_counter() - t
-    chosen, per_shape = b.tuning()
+    chosen, per_plan, shapes = b.tuning()
     times, kms = [], []
     for _ in range(max(1, a.steps // 2)):
         t = time.perf_counter()
@@ -137,6 +137,19 @@ def batch(a):
         times.append(time.perf_counter() - t)
         kms.append(ms)
     batch_s = min(times)
+    if nf.any():
+        raise RuntimeError(f"{int(nf.sum())} pairs failed the k-mer regression")
+    # spot check: jobs of the batch equal their single ppsk_dense calls (the
+    # largest sketch, the most k-mer lengths, the most queries, and two more)
+    def pick(key):
+        return max(range(len(jobs)), key=key)
+    check = sorted({pick(lambda x: views[x][0].s64), pick(lambda x: len(views[x][0].klist)),
+                    pick(lambda x: views[x][1].n), 0, len(jobs) // 2})
+    for x in check:
+        rv, qv = views[x]
+        want, _, _ = H.dense(rv, qv)
+        if not np.array_equal(b.result(x).view(np.uint8), want.view(np.uint8)):
+            raise RuntimeError(f"batch job {x} differs from its single ppsk_dense call")
     # plain path: one ppsk_dense call per job, output in device memory
     plain_s = 0.0
     for rv, qv in views:
@@ -159,8 +172,11 @@ def batch(a):
         "plain_value": round(pairs / plain_s, 1), "plain_s": round(plain_s, 4),
         "speedup_vs_plain": round(plain_s / batch_s, 2),
         "valu_frac": round(ops / (min(kms) * 1e-3) / 39.3216e12, 4),
-        "autotune": {"chosen_shape": chosen, "first_run_s": round(tune_s, 3),
-                     "shape_ms": {str(v): round(float(per_shape[v]), 2) for v in (2, 4)}},
+        "autotune": {"chosen_plan": chosen, "first_run_s": round(tune_s, 3),
+                     "plan_ms": {str(v): round(float(per_plan[v]), 2) for v in (2, 4, 5, 6)
+                                 if per_plan[v] > 0},
+                     "jobs_per_shape": {str(v): int((shapes == v).sum()) for v in (2, 4, 5)}},
+        "checked_jobs": check,
         "config": {"workload": "batched uneven query-vs-ref jobs", "jobs": len(jobs),
                    "pairs": pairs, "queries": "log-uniform 1..2048",
                    "refs": "uniform 500..20000", "classes": [
@@ -280,6 +296,96 @@ def serve(a):
                    "kmers": KL8, "bins": S64 * 64, "request_sizes": "log-uniform 1..512",
                    "kNN": 10, "threshold": 0.05, "slots": a.serve_slots, "max_rows_per_step": a.serve_max_rows,
                    "random_correction": "equal-base"},
+    }
+
+
+# ---------------------------------------------------------------------------
+def _poisson(srv, reqs, q_raw, q_le, qps, rng):
+    """Submits reqs at Poisson arrivals of rate qps while a consumer thread
+    collects the results; returns (results, wall seconds)."""
+    n_req = len(reqs)
+    gaps = rng.exponential(1.0 / qps, size=n_req)
+    tickets = [None] * n_req
+    results = [None] * n_req
+
+    def consumer():
+        for x in range(n_req):
+            while tickets[x] is None:
+                time.sleep(0.0005)
+            results[x] = srv.wait(tickets[x])
+
+    th = threading.Thread(target=consumer)
+    th.start()
+    t_begin = time.perf_counter()
+    nxt = t_begin
+    for x, (s_, e_) in enumerate(reqs):
+        nxt += gaps[x]
+        dt = nxt - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        tickets[x] = srv.submit(q_raw[s_:e_], q_le[s_:e_])
+    th.join()
+    return results, time.perf_counter() - t_begin
+
+
+def serve_headline(n_ref, seconds, max_rows, slots):
+    """The serving figure of the headline line: a resident database of n_ref
+    sketches (8 k-mer lengths, 10,000 bins), query batches of log-uniform
+    size in [1, 512], kNN = 10 and threshold 0.05 (core), equal-base
+    correction.  Saturation throughput (64 requests queued at once), then
+    Poisson arrivals at 70% of that request rate for `seconds`: p50 / p99 of
+    submit-to-result latency."""
+    n_pool = 4096
+    t0 = time.time()
+    raw, le, _ = H.synth(n_ref + n_pool, KL8, S64, seed=21, dcore_max=0.15)
+    q_raw, q_le = raw[n_ref:].copy(), le[n_ref:].copy()
+    ref = H.SketchSet(raw[:n_ref], KL8, S64, lengths=le[:n_ref])
+    del raw
+    setup_s = time.time() - t0
+    print(f"# serving: {n_ref} reference sketches generated and resident in {setup_s:.1f}s",
+          flush=True)
+    rng = np.random.default_rng(4)
+
+    def draw(m):
+        out = []
+        for _ in range(m):
+            size = min(512, max(1, int(math.floor(512.0 ** rng.uniform()))))
+            s = int(rng.integers(0, n_pool - size + 1))
+            out.append((s, s + size))
+        return out
+
+    srv = H.Server(ref, kNN=10, cutoff=0.05, random_mode=H.RANDOM_EQUAL_BASE,
+                   max_rows=max_rows, n_slots=slots)
+    try:
+        srv.wait(srv.submit(q_raw[:8], q_le[:8]))  # warm-up
+        sat_req = draw(64)
+        t = time.perf_counter()
+        tickets = [srv.submit(q_raw[s_:e_], q_le[s_:e_]) for s_, e_ in sat_req]
+        for tk in tickets:
+            srv.wait(tk)
+        sat_s = time.perf_counter() - t
+        sat_queries = sum(e_ - s_ for s_, e_ in sat_req)
+        sat_rate = sat_queries * n_ref / sat_s
+        sat_qps = len(sat_req) / sat_s
+        qps = 0.7 * sat_qps
+        reqs = draw(max(8, int(qps * seconds)))
+        results, wall = _poisson(srv, reqs, q_raw, q_le, qps, rng)
+    finally:
+        srv.free()
+        ref.free()
+    lat = np.array([(r["t_done"] - r["t_submit"]) * 1e3 for r in results])
+    pairs = sum(e_ - s_ for s_, e_ in reqs) * n_ref
+    return {
+        "p50_ms": round(float(np.percentile(lat, 50)), 2),
+        "p99_ms": round(float(np.percentile(lat, 99)), 2),
+        "value": round(pairs / wall, 1), "unit": "distances/s",
+        "saturation_value": round(sat_rate, 1),
+        "load": "Poisson arrivals at 70% of the saturation request rate",
+        "qps": round(qps, 3), "requests": len(reqs), "window_s": round(wall, 2),
+        "n_fail": int(sum(r["n_fail"] for r in results)),
+        "n_refs": n_ref, "kmers": KL8, "bins": S64 * 64,
+        "request_sizes": "log-uniform 1..512 sketches", "kNN": 10, "threshold": 0.05,
+        "slots": slots, "max_rows_per_step": max_rows, "random_correction": "equal-base",
     }
 
 
