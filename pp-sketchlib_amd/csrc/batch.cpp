@@ -153,7 +153,7 @@ int build_plan(ppsk_batch* b, int plan) {
     const bool self = jb.query == jb.ref;
     const int shape = plan_shape(b, plan, j);
     const int tr = ppsk::variant_tile_rows(shape), tc = ppsk::variant_tile_cols(shape);
-    auto tiles = make_tiles(jb.query->n, jb.ref->n, tr, tc, self, 0, jb.query->n, tile_group(), &gs);
+    auto tiles = make_tiles(jb.query->n, jb.ref->n, tr, tc, self, 0, jb.query->n, tile_group(tr), &gs);
     const size_t n = tiles.size();
     for (int qi = 0; qi < ppsk::BATCH_QUEUES; ++qi) {
       const size_t a = n * qi / ppsk::BATCH_QUEUES, e = n * (qi + 1) / ppsk::BATCH_QUEUES;

@@ -139,7 +139,11 @@ struct RandomIn {
 };
 
 int check_random(const ppsk_set* rows, const ppsk_set* cols, const RandomIn& rnd);
-int tile_group();
+// Row tiles per scheduling group for tiles of tile_rows rows: groups of 512
+// sample rows (8 tiles of 64 rows, 32 of 16), so that a column panel fetched
+// into an XCD's L2 serves the same number of rows whatever the tile shape;
+// PPSK_TILE_GROUP overrides the tile count.
+int tile_group(int tile_rows);
 int default_variant();
 
 // Timing of the tile-kernel launches of one call: per-chunk events on the

@@ -9,8 +9,9 @@
 // per 64-bin word per pair, which is the algorithmic op count.
 //
 // Tiling (GEMM-shaped, integer VALU, no MFMA):
-//  * a workgroup owns a (8 RQ) x (64 RR) tile of pairs for ONE k-mer length;
-//    8 waves x RQ rows each, RR columns per lane;
+//  * a workgroup owns a tile of pairs for ONE k-mer length: its 8 waves form
+//    a grid of row groups x column groups, each wave RQ rows x 64 RR columns
+//    (RR columns per lane);
 //  * the RQ row samples of a wave are wave-uniform, so their planes are read
 //    with scalar loads into SGPRs: each v_bitop3 takes its row operand from
 //    an SGPR;
@@ -172,11 +173,24 @@ __device__ __forceinline__ double jaccard_d(uint32_t c, double bins, double r) {
 // buffered (56 KiB per workgroup, two workgroups per CU).  Stages are copied
 // global -> LDS directly: round i of wave w moves uint4 [i*512 + w*64, +64)
 // of the stage; the last round is a half round.
+//
+// The 8 waves form a grid of RG row groups x CG column groups: wave w takes
+// rows (w / CG) * RQ .. +RQ and columns (w % CG) * 64 RR .. +64 RR of the
+// tile (lane l: columns l + 64 r, r < RR, of its group).  Every column
+// operand a lane reads from LDS meets RQ row operands, so the VALU work per
+// LDS byte is RQ / 4 ops: at RQ = 8 that matches the CU's VALU issue rate
+// (256 lane-ops per clock) to its LDS read rate (128 B per clock).  The
+// pipelined shapes therefore all keep RQ = 8 and span 256 columns
+// (CG = 4 / RR): 64 x 256 (8 x 1 waves), 32 x 256 (4 x 2), 16 x 256 (2 x 4).
+// The compiler-scheduled shapes (RQ = 16) stack all waves on the rows.
 template <int RQ, int RR>
 struct TileCfg {
-  static constexpr int TROWS = WAVES * RQ;
-  static constexpr int TC = 64 * RR;
-  static constexpr int WS = 4 / RR;
+  static constexpr int CG = RQ == 8 ? 4 / RR : 1;
+  static constexpr int RG = WAVES / CG;
+  static constexpr int TROWS = RG * RQ;
+  static constexpr int TC = CG * 64 * RR;
+  static constexpr int WS = 4 / (CG * RR);
+  static_assert(CG * RG == WAVES, "the wave grid covers the workgroup");
   static constexpr int STAGE_U4 = WS * HL * TC / 4;  // 1792
   static constexpr int ROUNDS = (STAGE_U4 + WAVES * 64 - 1) / (WAVES * 64);  // 4
   static_assert(STAGE_U4 % 64 == 0, "stage must be whole wave rounds");
@@ -322,6 +336,7 @@ __device__ __forceinline__ void match_stage_plain(const CU32* arow, int w0, int 
                                                   const uint32_t* sB, int lane,
                                                   uint32_t (&cnt)[RQ][RR]) {
   constexpr int TC = TileCfg<RQ, RR>::TC;
+  static_assert(TileCfg<RQ, RR>::CG == 1, "compiler-scheduled shapes stack waves on rows");
   for (int ws = 0; ws < nw; ++ws) {
     const CU32* aw = arow + (int64_t)(w0 + ws) * (HL * BLK);
     const uint32_t* bw = sB + ws * (HL * TC) + lane;
@@ -386,12 +401,29 @@ struct ATuple<2> {
 };
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
+// Column operands come in pairs from one ds_read2st64_b32: two column blocks
+// of one plane (RR >= 2), or one column of two consecutive planes (RR = 1).
+template <int RR, int PB>
+struct ColRegs {
+  static constexpr int N = RR >= 2 ? PB * (RR / 2) : PB / 2;
+};
+
 template <int RQ, int RR, int PB>
 struct Batch {
-  static_assert(RR % 2 == 0, "column operands are read in pairs");
-  typename ATuple<RQ>::T a[PB];  // row planes: RQ samples each
-  u32x2 b[PB][RR / 2];           // column planes: RR per lane
+  static_assert(RR >= 2 ? RR % 2 == 0 : PB % 2 == 0, "column operands are read in pairs");
+  typename ATuple<RQ>::T a[PB];     // row planes: RQ samples each
+  u32x2 b[ColRegs<RR, PB>::N];      // column planes: RR per lane and plane
 };
+
+// Column operand (plane pb of the batch, column block r) of a batch.
+template <int RQ, int RR, int PB>
+__device__ __forceinline__ uint32_t col_operand(const Batch<RQ, RR, PB>& x, int pb, int r) {
+  if constexpr (RR >= 2) {
+    return x.b[pb * (RR / 2) + r / 2][r % 2];
+  } else {
+    return x.b[pb / 2][pb % 2];
+  }
+}
 
 template <int RQ, int OFF>
 __device__ __forceinline__ void a_load(const CU32* aw, typename ATuple<RQ>::T& x) {
@@ -408,25 +440,36 @@ __device__ __forceinline__ void a_load(const CU32* aw, typename ATuple<RQ>::T& x
 }
 
 // Issue planes P0 .. P0+PB-1 of the word at (aw, bw); P0 = HL addresses the
-// first planes of the next word.  bw is the LDS byte address of plane 0,
-// column `lane` of the word; column r of plane P sits at bw + (P TC + 64 r) 4,
-// i.e. at offset P TC / 64 + r in the 256-byte units of ds_read2st64.
+// first planes of the next word.  bw is the LDS byte address of plane 0 at the
+// lane's first column (a stage holds TC columns per plane); column block r of
+// plane P sits at bw + (P TC + 64 r) 4, i.e. at offset P TC / 64 + r in the
+// 256-byte units of ds_read2st64.
 template <int RQ, int RR, int PB, int P0>
 __device__ __forceinline__ void batch_issue(const CU32* aw, uint32_t bw, Batch<RQ, RR, PB>& x) {
-  constexpr int TC = 64 * RR;
+  constexpr int TC = TileCfg<RQ, RR>::TC;  // plane stride of the LDS stage
 #pragma unroll
   for (int pb = 0; pb < PB; ++pb) {
     if (pb == 0) a_load<RQ, P0 * BLK * 4>(aw, x.a[0]);
     if (pb == 1) a_load<RQ, (P0 + 1) * BLK * 4>(aw, x.a[PB - 1]);
   }
+  if constexpr (RR >= 2) {
 #pragma unroll
-  for (int pb = 0; pb < PB; ++pb) {
+    for (int pb = 0; pb < PB; ++pb) {
 #pragma unroll
-    for (int rp = 0; rp < RR / 2; ++rp) {
+      for (int rp = 0; rp < RR / 2; ++rp) {
+        asm volatile("ds_read2st64_b32 %0, %1 offset0:%2 offset1:%3"
+                     : "=&v"(x.b[pb * (RR / 2) + rp])
+                     : "v"(bw), "i"((P0 + pb) * TC / 64 + 2 * rp),
+                       "i"((P0 + pb) * TC / 64 + 2 * rp + 1));
+      }
+    }
+  } else {
+    // one column: planes P0 + pb and P0 + pb + 1 in one read
+#pragma unroll
+    for (int pb = 0; pb < PB; pb += 2) {
       asm volatile("ds_read2st64_b32 %0, %1 offset0:%2 offset1:%3"
-                   : "=&v"(x.b[pb][rp])
-                   : "v"(bw), "i"((P0 + pb) * TC / 64 + 2 * rp),
-                     "i"((P0 + pb) * TC / 64 + 2 * rp + 1));
+                   : "=&v"(x.b[pb / 2])
+                   : "v"(bw), "i"((P0 + pb) * TC / 64), "i"((P0 + pb + 1) * TC / 64));
     }
   }
 }
@@ -435,16 +478,17 @@ template <int RQ, int RR, int PB>
 __device__ __forceinline__ void batch_wait(Batch<RQ, RR, PB>& x) {
   if constexpr (PB == 2 && RR == 4) {
     asm volatile("s_waitcnt lgkmcnt(0)"
-                 : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0][0]), "+v"(x.b[0][1]),
-                   "+v"(x.b[1][0]), "+v"(x.b[1][1]));
+                 : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0]), "+v"(x.b[1]), "+v"(x.b[2]),
+                   "+v"(x.b[3]));
   } else if constexpr (PB == 2 && RR == 2) {
-    asm volatile("s_waitcnt lgkmcnt(0)"
-                 : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0][0]), "+v"(x.b[1][0]));
+    asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0]), "+v"(x.b[1]));
+  } else if constexpr (PB == 2 && RR == 1) {
+    asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0]));
   } else if constexpr (PB == 1 && RR == 4) {
-    asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(x.a[0]), "+v"(x.b[0][0]), "+v"(x.b[0][1]));
+    asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(x.a[0]), "+v"(x.b[0]), "+v"(x.b[1]));
   } else {
     static_assert(PB == 1 && RR == 2, "unsupported batch shape");
-    asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(x.a[0]), "+v"(x.b[0][0]));
+    asm volatile("s_waitcnt lgkmcnt(0)" : "+s"(x.a[0]), "+v"(x.b[0]));
   }
 }
 
@@ -456,16 +500,17 @@ __device__ __forceinline__ void batch_wait(Batch<RQ, RR, PB>& x) {
 template <int RQ, int RR, int PB>
 __device__ __forceinline__ void batch_hold(Batch<RQ, RR, PB>& x) {
   if constexpr (PB == 2 && RR == 4) {
-    asm volatile(""
-                 : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0][0]), "+v"(x.b[0][1]),
-                   "+v"(x.b[1][0]), "+v"(x.b[1][1]));
+    asm volatile("" : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0]), "+v"(x.b[1]), "+v"(x.b[2]),
+                      "+v"(x.b[3]));
   } else if constexpr (PB == 2 && RR == 2) {
-    asm volatile("" : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0][0]), "+v"(x.b[1][0]));
+    asm volatile("" : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0]), "+v"(x.b[1]));
+  } else if constexpr (PB == 2 && RR == 1) {
+    asm volatile("" : "+s"(x.a[0]), "+s"(x.a[1]), "+v"(x.b[0]));
   } else if constexpr (PB == 1 && RR == 4) {
-    asm volatile("" : "+s"(x.a[0]), "+v"(x.b[0][0]), "+v"(x.b[0][1]));
+    asm volatile("" : "+s"(x.a[0]), "+v"(x.b[0]), "+v"(x.b[1]));
   } else {
     static_assert(PB == 1 && RR == 2, "unsupported batch shape");
-    asm volatile("" : "+s"(x.a[0]), "+v"(x.b[0][0]));
+    asm volatile("" : "+s"(x.a[0]), "+v"(x.b[0]));
   }
 }
 
@@ -479,7 +524,7 @@ __device__ __forceinline__ void match_batch(const Batch<RQ, RR, PB>& x, uint32_t
 #pragma unroll
       for (int r = 0; r < RR; ++r) {
         const uint32_t a = x.a[pb][q];
-        const uint32_t b = x.b[pb][r / 2][r % 2];
+        const uint32_t b = col_operand<RQ, RR, PB>(x, pb, r);
         if ((P + pb) % BBITS == 0) {
           acc[q][r] = ~(a ^ b);
         } else {
@@ -518,15 +563,16 @@ __device__ __forceinline__ void match_batches(const CU32* aw, uint32_t bw, bool 
   }
 }
 
+// col: the lane's first column within the tile ((wave % CG) * 64 RR + lane).
 template <int RQ, int RR, int PB>
 __device__ __forceinline__ void match_stage_pipe(const CU32* arow, int w0, int nw,
-                                                 const uint4* sbuf, int lane,
+                                                 const uint4* sbuf, int col,
                                                  uint32_t (&cnt)[RQ][RR]) {
   constexpr int TC = TileCfg<RQ, RR>::TC;
   static_assert((HL / PB) % 2 == 0, "an even batch count keeps the register sets aligned");
   Batch<RQ, RR, PB> c, n;
   const CU32* aw = arow + (int64_t)w0 * (HL * BLK);
-  uint32_t bw = lds_addr(sbuf) + lane * 4;
+  uint32_t bw = lds_addr(sbuf) + col * 4;
   batch_issue<RQ, RR, PB, 0>(aw, bw, c);
   for (int ws = 0; ws < nw; ++ws) {
     batch_wait<RQ, RR, PB>(c);
@@ -553,9 +599,12 @@ __device__ __noinline__ void tile_epilogue(CParams* pp, const uint16_t* tile_cou
   constexpr int TC = Cfg::TC;
   constexpr int TROWS = Cfg::TROWS;
   const double bins = 64.0 * (double)p.s64;
-  const int64_t gi0 = (int64_t)tile.bi * TROWS + wave * RQ;
-  const int64_t col0 = (int64_t)tile.ct * TC;
-  const uint16_t* tbase = tile_counts + (int64_t)(wave * RQ) * TC + lane;
+  // this wave's rows and this lane's first column in the tile (wave grid)
+  const int wrow = (wave / Cfg::CG) * RQ;
+  const int wcol = (wave % Cfg::CG) * 64 * RR + lane;
+  const int64_t gi0 = (int64_t)tile.bi * TROWS + wrow;
+  const int64_t col0 = (int64_t)tile.ct * TC + wcol - lane;
+  const uint16_t* tbase = tile_counts + (int64_t)wrow * TC + wcol;
   const int64_t kstride = (int64_t)TROWS * TC;
   for (int q = 0; q < RQ; ++q) {
     const int64_t gi = gi0 + q;
@@ -677,7 +726,9 @@ __device__ __forceinline__ void tile_match(CParams* pp, Tile tile, int kk, uint4
   const int64_t row0 = (int64_t)tile.bi * TROWS;
   const CU32* arow = (CU32*)(pp->row_stripes +
                              ((int64_t)kk * pp->nb_rows + row0 / BLK) * (int64_t)s64 * (HL * BLK) +
-                             row0 % BLK + wave * RQ);
+                             row0 % BLK + (wave / Cfg::CG) * RQ);
+  // the lane's first column within the tile
+  const int col_in_tile = (wave % Cfg::CG) * 64 * RR + lane;
   // column slabs of this k-mer length.  The last stage may run past word
   // s64 - 1: the stripe buffer carries zero padding at its end, and those
   // words are never computed on.
@@ -727,7 +778,7 @@ __device__ __forceinline__ void tile_match(CParams* pp, Tile tile, int kk, uint4
         if constexpr (PB == 0) {
           match_stage_plain<RQ, RR>(arow, w0, nw, sB, lane, cnt);
         } else {
-          match_stage_pipe<RQ, RR, PB>(arow, w0, nw, cur, lane, cnt);
+          match_stage_pipe<RQ, RR, PB>(arow, w0, nw, cur, col_in_tile, cnt);
         }
       }
     }
@@ -742,7 +793,8 @@ __device__ __forceinline__ void store_counts(uint16_t* tile_counts, int kk, int 
                                              const uint32_t (&cnt)[RQ][RR]) {
   using Cfg = TileCfg<RQ, RR>;
   constexpr int TC = Cfg::TC;
-  uint16_t* cbase = tile_counts + ((int64_t)kk * Cfg::TROWS + wave * RQ) * TC + lane;
+  uint16_t* cbase = tile_counts + ((int64_t)kk * Cfg::TROWS + (wave / Cfg::CG) * RQ) * TC +
+                    (wave % Cfg::CG) * 64 * RR + lane;
 #pragma unroll
   for (int q = 0; q < RQ; ++q)
 #pragma unroll
@@ -960,9 +1012,9 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_batch_mixed(BatchArgs a) {
     tile.ct = __builtin_amdgcn_readfirstlane(bi.ct);
     CParams* pp = (CParams*)(a.jobs + job);
     if (shape == 5) {
-      batch_tile<2, 4, 2>(pp, tile, scratch, a.ctl, wave, lane);
+      batch_tile<8, 1, 2>(pp, tile, scratch, a.ctl, wave, lane);
     } else if (shape == 4) {
-      batch_tile<4, 4, 2>(pp, tile, scratch, a.ctl, wave, lane);
+      batch_tile<8, 2, 2>(pp, tile, scratch, a.ctl, wave, lane);
     } else {
       batch_tile<8, 4, 2>(pp, tile, scratch, a.ctl, wave, lane);
     }
@@ -1018,10 +1070,10 @@ hipError_t launch_dists(const DistParams& p, int variant, int64_t n_items, hipSt
       dist_tiles<16, 2, 0><<<g, WAVES * 64, 0, stream>>>(p);
       break;
     case 4:
-      dist_tiles<4, 4, 2><<<g, WAVES * 64, 0, stream>>>(p);
+      dist_tiles<8, 2, 2><<<g, WAVES * 64, 0, stream>>>(p);
       break;
     case 5:
-      dist_tiles<2, 4, 2><<<g, WAVES * 64, 0, stream>>>(p);
+      dist_tiles<8, 1, 2><<<g, WAVES * 64, 0, stream>>>(p);
       break;
     default:
       dist_tiles<8, 4, 2><<<g, WAVES * 64, 0, stream>>>(p);
@@ -1036,7 +1088,7 @@ template <typename F>
 static hipError_t with_batch_kernel(int variant, F f) {
   switch (variant) {
     case 4:
-      return f(dist_batch<4, 4, 2>);
+      return f(dist_batch<8, 2, 2>);
     case 2:
       return f(dist_batch<8, 4, 2>);
     case BATCH_MIXED:

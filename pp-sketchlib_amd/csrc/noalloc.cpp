@@ -139,7 +139,7 @@ int ppsk_dense_noalloc(ppsk_set* ref, ppsk_set* query, int32_t random_mode, int3
   HIP_TRY(ppsk::launch_tile_reset(d_done, (int64_t)chunk, d_fail, c->stream));
   std::vector<size_t> gs;
   const std::vector<ppsk::Tile> tiles =
-      make_tiles(rows->n, cols->n, tr, tc, self, 0, rows->n, tile_group(), &gs);
+      make_tiles(rows->n, cols->n, tr, tc, self, 0, rows->n, tile_group(tr), &gs);
   Timing tm;
   for (size_t a = 0; a < tiles.size(); a += chunk) {
     const size_t b = std::min(tiles.size(), a + chunk);

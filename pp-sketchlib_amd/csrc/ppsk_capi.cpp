@@ -245,9 +245,9 @@ int check_random(const ppsk_set* rows, const ppsk_set* cols, const RandomIn& rnd
 
 // Row tiles per scheduling group (make_tiles); PPSK_TILE_GROUP overrides
 // (read on every call, so a process can compare settings).
-int tile_group() {
+int tile_group(int tile_rows) {
   const char* e = std::getenv("PPSK_TILE_GROUP");
-  const int g = e ? std::atoi(e) : 8;
+  const int g = e ? std::atoi(e) : 512 / (tile_rows > 0 ? tile_rows : 64);
   return g < 1 ? 1 : g;
 }
 
@@ -298,7 +298,7 @@ int run_dense_tiles(DeviceCtx* c, ppsk::DistParams p, int variant, bool upper, i
   }
   const size_t max_tiles = std::max<size_t>(1, budget / per_tile);
   const int64_t nct = (p.n_cols + tc - 1) / tc;
-  int group = tile_group();
+  int group = tile_group(tr);
   group = (int)std::max<int64_t>(1, std::min<int64_t>(group, (int64_t)max_tiles / nct));
   std::vector<size_t> gs;
   const std::vector<ppsk::Tile> tiles =
@@ -405,7 +405,7 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
   const bool finest = host_out && std::getenv("PPSK_CHUNK_BYTES");
   std::vector<size_t> gs;
   const std::vector<ppsk::Tile> tiles = make_tiles(p.n_rows, p.n_cols, tr, tc, upper, row_begin,
-                                                   row_end, finest ? 1 : tile_group(), &gs);
+                                                   row_end, finest ? 1 : tile_group(tr), &gs);
   if (tiles.empty()) return PPSK_OK;
   // 16-row tiles run on the mixed-shape kernel (every item in that shape)
   const int kernel = variant == 5 ? ppsk::BATCH_MIXED : variant;

@@ -112,10 +112,11 @@ hipError_t launch_stripe(const uint64_t* raw, int64_t count, int64_t first,
 
 // Tile kernel configurations ("variants"): rows per wave RQ (row operands in
 // SGPRs), columns per lane RR, row-operand pipeline PB (0 = left to the
-// compiler, 2 = explicit double-buffered scalar loads of 2 planes).
-//   1: RQ16 RR1 PB0 (128 x 64)     3: RQ16 RR2 PB0 (128 x 128)
-//   2: RQ8  RR4 PB2 (64 x 256)     4: RQ4  RR4 PB2 (32 x 256)
-//   5: RQ2  RR4 PB2 (16 x 256)
+// compiler, 2 = explicit double-buffered scalar loads of 2 planes); the 8
+// waves of a workgroup form a grid of row groups x column groups.
+//   1: RQ16 RR1 PB0, waves 8 x 1 (128 x 64)   3: RQ16 RR2 PB0, 8 x 1 (128 x 128)
+//   2: RQ8  RR4 PB2, waves 8 x 1 (64 x 256)   4: RQ8  RR2 PB2, 4 x 2 (32 x 256)
+//   5: RQ8  RR1 PB2, waves 2 x 4 (16 x 256)
 // A pipelined (PB = 2) configuration is only correct if the compiler keeps
 // the in-flight SGPR tuples in place; the build rejects any that spill.
 constexpr int DEFAULT_VARIANT = 2;

@@ -370,7 +370,7 @@ int run_step(ppsk_server* s, Worker& w, std::vector<Piece>& batch, int64_t rows,
   const int tr = ppsk::variant_tile_rows(variant);
   const int tc = ppsk::variant_tile_cols(variant);
   std::vector<size_t> gs;
-  const auto tiles = make_tiles(rows, ref->n, tr, tc, false, 0, rows, tile_group(), &gs);
+  const auto tiles = make_tiles(rows, ref->n, tr, tc, false, 0, rows, tile_group(tr), &gs);
   if (L.persistent) {
     // whole tiles from 8 queues, each a contiguous eighth of the tile list
     // less its tail
