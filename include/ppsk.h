@@ -268,26 +268,28 @@ int ppsk_batch_create(int device_id, const ppsk_job *jobs, int32_t n_jobs,
 /*
  * Run every job: one graph launch (queue reset + persistent kernel), replayed
  * on later runs.  variant: the plan -- every job in one tile shape (2:
- * 64 x 256, 4: 32 x 256, 5: 16 x 256), 6: each job in its own shape (one
- * persistent launch running several shapes), or 0 for the autotuner.  On the
- * first run of a batch with an unseen job-shape signature the autotuner
- * times plans 2, 4 and 5 on the batch itself, derives each shape's
- * throughput on padded work, gives every job the shape of least estimated
- * time, times that plan 6 too and keeps the fastest plan for the process.
- * Before tuning, plan 6 gives each job the shape with the least padding.
- * n_fail (may be NULL): per-job count of pairs whose regression failed
- * (PPSK_ERR_FIT is returned after all output is written).  used_variant /
- * kernel_ms (may be NULL): plan used and its launch time.
+ * 64 x 256, 4: 32 x 256, 5: 16 x 256), 6: each job's rows split over the
+ * three tile heights (row tiles laid from its first row, each at a multiple
+ * of its own height; one persistent launch running every shape), or 0 for
+ * the autotuner.  On the first run of a batch with an unseen job-shape
+ * signature the autotuner times plans 2, 4 and 5 on the batch itself,
+ * derives each shape's throughput on padded work, splits every job's rows
+ * for the least estimated time, times that plan 6 too and keeps the fastest
+ * plan for the process.  Before tuning, plan 6 splits each job's rows for
+ * the least padding.  n_fail (may be NULL): per-job count of pairs whose
+ * regression failed (PPSK_ERR_FIT is returned after all output is written).
+ * used_variant / kernel_ms (may be NULL): plan used and its launch time.
  */
 int ppsk_batch_run(ppsk_batch *batch, int32_t variant, uint64_t *n_fail,
                    int32_t *used_variant, double *kernel_ms);
 
 /* Autotuner result: chosen plan and per-plan time (index = plan, 0..6;
- * 0 where not timed); shape_per_job (may be NULL, n_jobs entries): each
- * job's tile shape in plan 6 (after tuning: the shape of least estimated
- * time; plan 6 is not timed when that is one shape for every job). */
+ * 0 where not timed); row_tiles (may be NULL, 3 x n_jobs entries): for each
+ * job, the row tiles of 64, 32 and 16 rows of its plan-6 split (after
+ * tuning: the split of least estimated time; plan 6 is not timed when that
+ * is one shape for every job). */
 int ppsk_batch_tuning(const ppsk_batch *batch, int32_t *variant,
-                      double *ms_per_variant, int32_t *shape_per_job);
+                      double *ms_per_variant, int32_t *row_tiles);
 
 int ppsk_batch_free(ppsk_batch *batch);
 

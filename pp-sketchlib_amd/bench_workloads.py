@@ -129,7 +129,7 @@ def batch(a):
     t = time.perf_counter()
     b.run()  # first run: the autotuner times each tile shape on this batch
     tune_s = time.perf_counter() - t
-    chosen, per_plan, shapes = b.tuning()
+    chosen, per_plan, row_tiles = b.tuning()
     times, kms = [], []
     for _ in range(max(1, a.steps // 2)):
         t = time.perf_counter()
@@ -175,7 +175,8 @@ def batch(a):
         "autotune": {"chosen_plan": chosen, "first_run_s": round(tune_s, 3),
                      "plan_ms": {str(v): round(float(per_plan[v]), 2) for v in (2, 4, 5, 6)
                                  if per_plan[v] > 0},
-                     "jobs_per_shape": {str(v): int((shapes == v).sum()) for v in (2, 4, 5)}},
+                     "row_tiles_per_height": {str(h): int(row_tiles[:, x].sum())
+                                              for x, h in enumerate((64, 32, 16))}},
         "checked_jobs": check,
         "config": {"workload": "batched uneven query-vs-ref jobs", "jobs": len(jobs),
                    "pairs": pairs, "queries": "log-uniform 1..2048",

@@ -5,12 +5,13 @@
 //   2, 4  every job in 64 x 256 / 32 x 256 tiles, on that shape's own
 //         persistent kernel;
 //   5     every job in 16 x 256 tiles, on the mixed-shape kernel;
-//   6     each job in its own shape (per-job choice below), on the
-//         mixed-shape kernel.
+//   6     each job's rows split over the three tile heights (row tiles laid
+//         from its first row, e.g. 15 x 64 + 32 + 16 rows for 1,000
+//         queries), on the mixed-shape kernel.
 // The autotuner times plans 2, 4 and 5 on the batch itself, derives each
-// shape's throughput on padded work from those runs, gives every job the
-// shape with the least estimated time (padded work / throughput), times that
-// per-job plan too and keeps the fastest of the four.
+// shape's throughput on padded work from those runs, splits every job's rows
+// for the least estimated time (padded work / throughput), times that plan
+// too and keeps the fastest of the four.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,6 +28,18 @@ using namespace ppsk::capi;
 // ---------------------------------------------------------------------------
 // Batched jobs: one persistent launch over many query-vs-ref jobs.
 // ---------------------------------------------------------------------------
+namespace {
+// Rows [row_begin, row_end) of a job in row tiles of one shape.
+struct Seg {
+  int shape;
+  int64_t row_begin, row_end;
+  bool operator==(const Seg& o) const {
+    return shape == o.shape && row_begin == o.row_begin && row_end == o.row_end;
+  }
+};
+using Split = std::vector<Seg>;
+}  // namespace
+
 struct ppsk_batch {
   struct Plan {  // one tile shape
     ppsk::BatchItem* items = nullptr;
@@ -50,7 +63,7 @@ struct ppsk_batch {
   Plan plans[ppsk::BATCH_MIXED + 1];
   int variant = 0;                      // chosen plan (0: not chosen yet)
   double tune_ms[ppsk::BATCH_MIXED + 1] = {};
-  std::vector<int> job_shape;           // plan 6: tile shape per job
+  std::vector<Split> job_split;         // plan 6: row split per job
   uint64_t signature = 0;
 };
 
@@ -64,50 +77,103 @@ const int kUniformPlans[] = {2, 4, 5};
 struct Tuned {
   uint64_t signature;
   int plan;
-  std::vector<int> job_shape;
+  std::vector<Split> job_split;
 };
 std::mutex g_tune_mu;
-std::vector<Tuned> g_tuned;  // job-shape signature -> plan (and per-job shapes)
+std::vector<Tuned> g_tuned;  // job-shape signature -> plan (and per-job splits)
 
 // Kernel that runs a plan.
 int plan_kernel(int plan) { return plan == 2 || plan == 4 ? plan : ppsk::BATCH_MIXED; }
 
-// Tile shape of job j in a plan.
-int plan_shape(const ppsk_batch* b, int plan, int j) {
-  return plan == ppsk::BATCH_MIXED ? b->job_shape[j] : plan;
+// Row segments of job j in a plan.
+Split plan_split(const ppsk_batch* b, int plan, int j) {
+  if (plan == ppsk::BATCH_MIXED) return b->job_split[j];
+  return {{plan, 0, b->jobs[j].query->n}};
 }
 
-// Tiles of job j in tile shape v (padding included).
-int64_t job_tiles(const ppsk_job& jb, int v) {
-  const int tr = ppsk::variant_tile_rows(v), tc = ppsk::variant_tile_cols(v);
-  const int64_t nbr = (jb.query->n + tr - 1) / tr, nct = (jb.ref->n + tc - 1) / tc;
-  if (jb.query != jb.ref) return nbr * nct;
-  int64_t t = 0;  // self job: upper-triangle tiles, as make_tiles keeps them
-  for (int64_t ct = 0; ct < nct; ++ct) t += std::min<int64_t>(nbr, ((ct + 1) * tc - 1 + tr - 1) / tr);
+// Column tiles kept for the row tile starting at row0 (self jobs: the upper
+// triangle, as make_tiles keeps it: (ct + 1) * tc - 1 > row0).
+int64_t kept_cols(const ppsk_job& jb, int v, int64_t row0) {
+  const int tc = ppsk::variant_tile_cols(v);
+  const int64_t nct = (jb.ref->n + tc - 1) / tc;
+  if (jb.query != jb.ref) return nct;
+  return nct - std::min<int64_t>(nct, (row0 + 1) / tc);
+}
+
+// Tiles of one segment (padding included).
+int64_t seg_tiles(const ppsk_job& jb, const Seg& s) {
+  const int tr = ppsk::variant_tile_rows(s.shape);
+  const int64_t b0 = s.row_begin / tr, b1 = (s.row_end + tr - 1) / tr;
+  if (jb.query != jb.ref) return (b1 - b0) * kept_cols(jb, s.shape, 0);
+  int64_t t = 0;
+  for (int64_t bi = b0; bi < b1; ++bi) t += kept_cols(jb, s.shape, bi * tr);
   return t;
 }
 
-// int32 matching operations of job j's padded tiles in shape v.
-double padded_ops(const ppsk_job& jb, int v) {
-  return (double)job_tiles(jb, v) * ppsk::variant_tile_rows(v) * ppsk::variant_tile_cols(v) *
-         jb.ref->n_k * jb.ref->s64 * 30.0;
+int64_t plan_tiles(const ppsk_batch* b, int plan, int j) {
+  int64_t t = 0;
+  for (const Seg& s : plan_split(b, plan, j)) t += seg_tiles(b->jobs[j], s);
+  return t;
 }
 
-// Per-job shapes: least padded work / shape throughput (ops per ms).  With
-// no measured throughputs (rate <= 0) all shapes count as equally fast.
-std::vector<int> choose_shapes(const ppsk_batch* b, const double* rate) {
-  std::vector<int> s(b->n_jobs, 2);
-  for (int j = 0; j < b->n_jobs; ++j) {
-    double best = 0.0;
-    for (int v : kUniformPlans) {  // larger tiles first: they win ties
+// int32 matching operations of job j's padded tiles in a plan.
+double plan_ops(const ppsk_batch* b, int plan, int j) {
+  const ppsk_job& jb = b->jobs[j];
+  double ops = 0.0;
+  for (const Seg& s : plan_split(b, plan, j)) {
+    ops += (double)seg_tiles(jb, s) * ppsk::variant_tile_rows(s.shape) *
+           ppsk::variant_tile_cols(s.shape);
+  }
+  return ops * jb.ref->n_k * jb.ref->s64 * 30.0;
+}
+
+// Row split of one job for plan 6: a sequence of row tiles of the uniform
+// shapes, each starting at a multiple of its own height, covering the job's
+// rows with the least estimated time (tile work / shape throughput in ops
+// per ms).  Dynamic programming over the rows in steps of the smallest tile
+// height, from the last row up; ties go to the taller tile, so the split is
+// taller tiles first.  With no measured throughputs (rate <= 0) every shape
+// counts as equally fast: the split of least padding.
+Split choose_split(const ppsk_job& jb, const double* rate) {
+  const int unit = ppsk::variant_tile_rows(5);
+  const int64_t n = jb.query->n, steps = (n + unit - 1) / unit;
+  const double per_pair = (double)jb.ref->n_k * jb.ref->s64 * 30.0;
+  std::vector<double> cost(steps + 1, 0.0);
+  std::vector<int> pick(steps, 2);
+  for (int64_t p = steps - 1; p >= 0; --p) {
+    double best = -1.0;
+    for (int v : kUniformPlans) {  // tallest first: it wins ties
+      const int tr = ppsk::variant_tile_rows(v), tc = ppsk::variant_tile_cols(v);
+      const int64_t len = tr / unit;
+      if (p % len) continue;
       const double r = rate[v] > 0 ? rate[v] : 1.0;
-      const double t = padded_ops(b->jobs[j], v) / r;
-      if (v == 2 || t < best) {
-        best = t;
-        s[j] = v;
+      const double c = (double)kept_cols(jb, v, p * unit) * tr * tc * per_pair / r +
+                       cost[std::min(steps, p + len)];
+      if (best < 0.0 || c < best) {
+        best = c;
+        pick[p] = v;
       }
     }
+    cost[p] = best;
   }
+  Split out;
+  for (int64_t p = 0; p < steps;) {
+    const int v = pick[p];
+    const int64_t len = ppsk::variant_tile_rows(v) / unit;
+    const int64_t r0 = p * unit, r1 = std::min(n, (p + len) * unit);
+    if (!out.empty() && out.back().shape == v) {
+      out.back().row_end = r1;
+    } else {
+      out.push_back({v, r0, r1});
+    }
+    p += len;
+  }
+  return out;
+}
+
+std::vector<Split> choose_splits(const ppsk_batch* b, const double* rate) {
+  std::vector<Split> s(b->n_jobs);
+  for (int j = 0; j < b->n_jobs; ++j) s[j] = choose_split(b->jobs[j], rate);
   return s;
 }
 
@@ -128,36 +194,42 @@ uint64_t batch_signature(const ppsk_batch* b) {
   return h;
 }
 
-// Items of one tile shape.  Every job's tiles (grouped order) are cut into
-// BATCH_QUEUES contiguous runs, run q of every job going to queue q, so each
-// queue (and the XCD that drains it) gets an equal share of every job and
-// walks compact blocks of it.  Jobs enter the queues most expensive tile
-// first, leaving the cheap tiles for the end of the launch.
+// Items of a plan.  Every (job, row segment) unit's tiles (grouped order) are
+// cut into BATCH_QUEUES contiguous runs, run q of every unit going to queue
+// q, so each queue (and the XCD that drains it) gets an equal share of every
+// job and walks compact blocks of it.  Units enter the queues most expensive
+// tile first, leaving the cheap tiles for the end of the launch.
 int build_plan(ppsk_batch* b, int plan) {
   auto& pl = b->plans[plan];
   if (pl.exec) return PPSK_OK;
   const int kernel = plan_kernel(plan);
-  // most expensive tiles first: the cheap ones fill the end of the launch
-  auto tile_cost = [&](int j) {
-    return (int64_t)b->jobs[j].ref->n_k * b->jobs[j].ref->s64 *
-           ppsk::variant_tile_rows(plan_shape(b, plan, j));
+  struct Unit {
+    int job;
+    Seg seg;
+    int64_t cost;  // matching work of one tile
   };
-  std::vector<int> order(b->n_jobs);
-  for (int j = 0; j < b->n_jobs; ++j) order[j] = j;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int x, int y) { return tile_cost(x) > tile_cost(y); });
+  std::vector<Unit> units;
+  for (int j = 0; j < b->n_jobs; ++j) {
+    for (const Seg& sg : plan_split(b, plan, j)) {
+      units.push_back({j, sg, (int64_t)b->jobs[j].ref->n_k * b->jobs[j].ref->s64 *
+                                  ppsk::variant_tile_rows(sg.shape)});
+    }
+  }
+  std::stable_sort(units.begin(), units.end(),
+                   [](const Unit& x, const Unit& y) { return x.cost > y.cost; });
   std::vector<std::vector<ppsk::BatchItem>> q(ppsk::BATCH_QUEUES);
   std::vector<size_t> gs;
-  for (int j : order) {
-    const ppsk_job& jb = b->jobs[j];
+  for (const Unit& u : units) {
+    const ppsk_job& jb = b->jobs[u.job];
     const bool self = jb.query == jb.ref;
-    const int shape = plan_shape(b, plan, j);
+    const int shape = u.seg.shape;
     const int tr = ppsk::variant_tile_rows(shape), tc = ppsk::variant_tile_cols(shape);
-    auto tiles = make_tiles(jb.query->n, jb.ref->n, tr, tc, self, 0, jb.query->n, tile_group(tr), &gs);
+    auto tiles = make_tiles(jb.query->n, jb.ref->n, tr, tc, self, u.seg.row_begin,
+                            u.seg.row_end, tile_group(tr), &gs);
     const size_t n = tiles.size();
     for (int qi = 0; qi < ppsk::BATCH_QUEUES; ++qi) {
       const size_t a = n * qi / ppsk::BATCH_QUEUES, e = n * (qi + 1) / ppsk::BATCH_QUEUES;
-      for (size_t x = a; x < e; ++x) q[qi].push_back({j, tiles[x].bi, tiles[x].ct, shape});
+      for (size_t x = a; x < e; ++x) q[qi].push_back({u.job, tiles[x].bi, tiles[x].ct, shape});
     }
   }
   std::vector<ppsk::BatchItem> all;
@@ -216,7 +288,7 @@ int run_plan(ppsk_batch* b, int plan, double* ms) {
   if (int rc = cancel_status(b->device)) return rc;
   uint64_t items = 0;
   for (int j = 0; j < b->n_jobs; ++j) {
-    items += (uint64_t)job_tiles(b->jobs[j], plan_shape(b, plan, j)) * b->jobs[j].ref->n_k;
+    items += (uint64_t)plan_tiles(b, plan, j) * b->jobs[j].ref->n_k;
   }
   CallScope scope(c, items);
   hipEvent_t e0 = pool_event(c, 0), e1 = pool_event(c, 1);
@@ -306,7 +378,7 @@ int ppsk_batch_create(int device_id, const ppsk_job* jobs, int32_t n_jobs, ppsk_
   }
   b->signature = batch_signature(b);
   const double none[ppsk::BATCH_MIXED + 1] = {};
-  b->job_shape = choose_shapes(b, none);  // least padding until tuned
+  b->job_split = choose_splits(b, none);  // least padding until tuned
   *out = b;
   return PPSK_OK;
 }
@@ -337,7 +409,7 @@ int ppsk_batch_run(ppsk_batch* b, int32_t variant, uint64_t* n_fail, int32_t* us
       if (t.signature == b->signature) {
         b->variant = t.plan;
         if (t.plan == ppsk::BATCH_MIXED && b->plans[t.plan].exec == nullptr) {
-          b->job_shape = t.job_shape;
+          b->job_split = t.job_split;
         }
       }
     }
@@ -345,8 +417,8 @@ int ppsk_batch_run(ppsk_batch* b, int32_t variant, uint64_t* n_fail, int32_t* us
   if (v == 0 && b->variant == 0) {
     // autotune: each uniform shape runs the whole batch twice (the first run
     // pays for graph set-up and cold caches) and its second run gives the
-    // shape's throughput on padded work; the per-job plan built from those
-    // throughputs is timed the same way; the fastest plan wins.  Each run
+    // shape's throughput on padded work; the row splits built from those
+    // throughputs are timed the same way; the fastest plan wins.  Each run
     // produces the complete, correct output.
     double rate[ppsk::BATCH_MIXED + 1] = {};
     auto time_plan = [&](int plan) -> int {
@@ -360,11 +432,11 @@ int ppsk_batch_run(ppsk_batch* b, int32_t variant, uint64_t* n_fail, int32_t* us
     for (int cand : kUniformPlans) {
       if (int rc = time_plan(cand)) return rc;
       double ops = 0.0;
-      for (const ppsk_job& jb : b->jobs) ops += padded_ops(jb, cand);
+      for (int j = 0; j < b->n_jobs; ++j) ops += plan_ops(b, cand, j);
       rate[cand] = b->tune_ms[cand] > 0 ? ops / b->tune_ms[cand] : 0.0;
     }
-    std::vector<int> shapes = choose_shapes(b, rate);
-    if (b->plans[ppsk::BATCH_MIXED].exec && shapes != b->job_shape) {
+    std::vector<Split> splits = choose_splits(b, rate);
+    if (b->plans[ppsk::BATCH_MIXED].exec && splits != b->job_split) {
       // a per-job plan captured with other shapes: rebuilt on its next use
       auto& pl = b->plans[ppsk::BATCH_MIXED];
       hipGraphExecDestroy(pl.exec);
@@ -372,15 +444,17 @@ int ppsk_batch_run(ppsk_batch* b, int32_t variant, uint64_t* n_fail, int32_t* us
       hipFree(pl.items);
       pl = ppsk_batch::Plan();
     }
-    b->job_shape = shapes;
-    // one shape for every job: plan 6 is that uniform plan, already timed
-    const bool uniform = std::all_of(shapes.begin(), shapes.end(),
-                                     [&](int x) { return x == shapes[0]; });
+    b->job_split = splits;
+    // every job in one segment of one shape: plan 6 is that uniform plan,
+    // already timed
+    const bool uniform = std::all_of(splits.begin(), splits.end(), [&](const Split& x) {
+      return x.size() == 1 && x[0].shape == splits[0][0].shape;
+    });
     if (!uniform) {
       if (int rc = time_plan(ppsk::BATCH_MIXED)) return rc;
     }
     std::lock_guard<std::mutex> tl(g_tune_mu);
-    g_tuned.push_back({b->signature, b->variant, b->job_shape});
+    g_tuned.push_back({b->signature, b->variant, b->job_split});
     v = b->variant;
     ms = b->tune_ms[v];
   } else {
@@ -405,10 +479,20 @@ int ppsk_batch_run(ppsk_batch* b, int32_t variant, uint64_t* n_fail, int32_t* us
 }
 
 int ppsk_batch_tuning(const ppsk_batch* b, int32_t* variant, double* ms_per_variant,
-                      int32_t* shape_per_job) {
+                      int32_t* row_tiles) {
   if (!b) return fail(PPSK_ERR_ARG, "batch is NULL");
   if (variant) *variant = b->variant;
-  if (shape_per_job) std::copy(b->job_shape.begin(), b->job_shape.end(), shape_per_job);
+  if (row_tiles) {
+    for (int j = 0; j < b->n_jobs; ++j) {
+      int32_t* t = row_tiles + 3 * (size_t)j;
+      t[0] = t[1] = t[2] = 0;
+      for (const Seg& sg : b->job_split[j]) {
+        const int tr = ppsk::variant_tile_rows(sg.shape);
+        const int x = sg.shape == 2 ? 0 : (sg.shape == 4 ? 1 : 2);
+        t[x] += (int32_t)((sg.row_end + tr - 1) / tr - sg.row_begin / tr);
+      }
+    }
+  }
   if (ms_per_variant) {
     for (int v = 0; v <= ppsk::BATCH_MIXED; ++v) ms_per_variant[v] = b->tune_ms[v];
   }

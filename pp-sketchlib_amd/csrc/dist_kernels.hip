@@ -969,10 +969,10 @@ __device__ __noinline__ void batch_tile(CParams* pp, Tile tile, uint16_t* scratc
 }
 
 // The persistent kernel of dist_batch with a tile shape per item (item.pad:
-// 2 = 64 x 256, 4 = 32 x 256, 5 = 16 x 256), so that each job of a batch runs
-// in the shape that wastes the least of its work on padding rows.  The
-// shapes share the LDS stage buffers; a workgroup's scratch block holds the
-// counts of the largest shape.
+// 2 = 64 x 256, 4 = 32 x 256, 5 = 16 x 256), so that each job of a batch can
+// cover its rows with the tile heights that waste the least of its work on
+// padding rows.  The shapes share the LDS stage buffers; a workgroup's
+// scratch block holds the counts of the largest shape.
 __global__ __launch_bounds__(WAVES * 64, 4) void dist_batch_mixed(BatchArgs a) {
   __shared__ int64_t s_item;
   __shared__ int s_tried;

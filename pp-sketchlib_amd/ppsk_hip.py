@@ -301,12 +301,13 @@ class Batch:
         return nf, used.value, ms.value
 
     def tuning(self):
-        """(chosen plan, time per plan [0..6], per-job tile shape of plan 6)."""
+        """(chosen plan, time per plan [0..6], row tiles of plan 6: an
+        (n_jobs, 3) array of each job's row tiles of 64, 32 and 16 rows)."""
         v = _i32()
         ms = np.zeros(7, np.float64)
-        shapes = np.zeros(len(self.specs), np.int32)
-        _check(lib().ppsk_batch_tuning(self._h, _c.byref(v), _ptr(ms), _ptr(shapes)))
-        return v.value, ms, shapes
+        tiles = np.zeros((len(self.specs), 3), np.int32)
+        _check(lib().ppsk_batch_tuning(self._h, _c.byref(v), _ptr(ms), _ptr(tiles)))
+        return v.value, ms, tiles
 
     def result(self, x):
         dptr, shape, dt = self.specs[x]

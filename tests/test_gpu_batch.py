@@ -6,7 +6,8 @@ subset is checked against the oracle (counts and Jaccards bit-exact,
 core/accessory within 1e-6).  Jobs mix k-sets (including k >= 33, where only
 no correction or a Monte Carlo table applies), sketch sizes, sample counts
 from 1 query upwards, random modes and output kinds, and query prefixes of
-resident databases.
+resident databases.  Self jobs (query set = reference set) give the
+condensed upper triangle, as ppsk_dense with no query set does.
 """
 import numpy as np
 import pytest
@@ -14,6 +15,7 @@ import pytest
 import oracle as O
 import ppsk_hip as H
 from helpers import mc_table
+import functools
 
 pytestmark = pytest.mark.gpu
 
@@ -26,7 +28,8 @@ CLASSES = [
     ([15, 17, 19, 21, 23, 25, 27, 29], 156, 600, 130),
     ([21, 24, 27, 30, 33, 36, 39], 30, 900, 40),
 ]
-# class, refs, queries, random correction, output
+# class, refs, queries (0: self job, the refs against themselves), random
+# correction, output
 JOBS = [
     (0, 700, 1, "none", "core_acc"),
     (0, 800, 33, "equal", "jaccard"),
@@ -38,8 +41,10 @@ JOBS = [
     (3, 129, 1, "none", "jaccard"),
     (1, 300, 65, "none", "counts"),
     (0, 1, 7, "none", "core_acc"),
+    (2, 300, 0, "equal", "core_acc"),
+    (1, 590, 0, "mc", "jaccard"),
 ]
-ORACLE_JOBS = [0, 1, 3, 5, 6, 7, 9]
+ORACLE_JOBS = [0, 1, 3, 5, 6, 7, 9, 10, 11]
 
 
 def _mode(name):
@@ -71,12 +76,14 @@ def dbs():
 def _job_sets(dbs, spec):
     ci, nr, nq, rnd, kind = spec
     d = dbs[ci]
-    return d, d["ref"].prefix(nr), d["qry"].prefix(nq)
+    ref = d["ref"].prefix(nr)
+    return d, ref, (ref if nq == 0 else d["qry"].prefix(nq))
 
 
 def _single(d, ref, qry, rnd, kind):
     table = d["table"] if rnd == "mc" else None
-    out, nf, _ = H.dense(ref, qry, random_mode=_mode(rnd), table=table, out=kind)
+    out, nf, _ = H.dense(ref, None if qry is ref else qry, random_mode=_mode(rnd), table=table,
+                         out=kind)
     return out, nf
 
 
@@ -94,30 +101,67 @@ def batch_run(dbs):
     b.free()
 
 
-def _padded_ops(spec, v):
-    """Matching ops of a job's tiles in shape v (64, 32 or 16 x 256), padding
-    included: query-vs-ref jobs cover every (row tile, column tile)."""
-    ci, nr, nq, _, _ = spec
+TILE_ROWS = {2: 64, 4: 32, 5: 16}
+
+
+def _kept_cols(spec, row0):
+    """Column tiles (256 wide) of the row tile starting at row0: all of them
+    for a query job, those reaching past the diagonal for a self job."""
+    _, nr, nq, _, _ = spec
+    nct = -(-nr // 256)
+    return nct if nq else nct - min(nct, (row0 + 1) // 256)
+
+
+def _tile_ops(spec, v, row0):
+    ci = spec[0]
     kl, s64 = CLASSES[ci][0], CLASSES[ci][1]
-    tr = {2: 64, 4: 32, 5: 16}[v]
-    tiles = -(-nq // tr) * -(-nr // 256)
-    return tiles * tr * 256 * len(kl) * s64 * 30
+    return _kept_cols(spec, row0) * TILE_ROWS[v] * 256 * len(kl) * s64 * 30
+
+
+def _uniform_ops(spec, v):
+    """Matching ops of a job's tiles in shape v, padding rows included."""
+    rows = spec[2] or spec[1]
+    tr = TILE_ROWS[v]
+    return sum(_tile_ops(spec, v, b * tr) for b in range(-(-rows // tr)))
+
+
+def _best_split_time(spec, rate):
+    """Least estimated time over every way to lay row tiles of 64, 32 and 16
+    rows from row 0 (each at a multiple of its height) until the job's rows
+    are covered: brute force over all such sequences."""
+    rows = spec[2] or spec[1]
+
+    @functools.lru_cache(maxsize=None)
+    def best(row0):
+        if row0 >= rows:
+            return 0.0
+        return min(_tile_ops(spec, v, row0) / rate[v] + best(row0 + tr)
+                   for v, tr in TILE_ROWS.items() if row0 % tr == 0)
+    return best(0)
 
 
 def test_batch_autotuned_equals_single_calls(batch_run):
     b, views = batch_run
     nf, used, ms = b.run()  # first run: autotuner times every shape
     assert used in (2, 4, 5, 6) and ms > 0
-    chosen, per, shapes = b.tuning()
+    chosen, per, row_tiles = b.tuning()
     assert chosen == used and per[2] > 0 and per[4] > 0 and per[5] > 0
-    # the per-job shapes follow the tuner's model: padded work of the job in a
-    # shape / that shape's measured throughput on the whole batch's padded work
-    ops = {v: [_padded_ops(spec, v) for spec in JOBS] for v in (2, 4, 5)}
-    rate = {v: sum(ops[v]) / per[v] for v in (2, 4, 5)}
-    for x in range(len(JOBS)):
-        est = {v: ops[v][x] / rate[v] for v in (2, 4, 5)}
-        best = min(est.values())
-        assert est[int(shapes[x])] == best, (x, est, shapes[x])
+    # each job's split follows the tuner's model: padded work of a tile /
+    # that shape's measured throughput on the whole batch's padded work,
+    # least in total over every possible split (brute force here)
+    rate = {v: sum(_uniform_ops(spec, v) for spec in JOBS) / per[v] for v in (2, 4, 5)}
+    for x, spec in enumerate(JOBS):
+        rows = spec[2] or spec[1]
+        n64, n32, n16 = (int(t) for t in row_tiles[x])
+        assert 64 * n64 + 32 * n32 + 16 * n16 >= rows, (x, row_tiles[x])
+        # the split lays the tallest tiles first
+        r, est = 0, 0.0
+        for v, n in ((2, n64), (4, n32), (5, n16)):
+            for _ in range(n):
+                est += _tile_ops(spec, v, r) / rate[v]
+                r += TILE_ROWS[v]
+        want = _best_split_time(spec, rate)
+        assert est == pytest.approx(want, rel=1e-9), (x, row_tiles[x], est, want)
     assert (nf == 0).all()
     for x, spec in enumerate(JOBS):
         d, ref, qry = views[x]
@@ -131,7 +175,8 @@ def test_batch_autotuned_equals_single_calls(batch_run):
 @pytest.mark.parametrize("variant", [2, 4, 5, 6])
 def test_batch_each_shape_and_graph_replays(batch_run, variant):
     # plans 2 / 4: every job in one shape on its own kernel; 5: every job in
-    # 16-row tiles and 6: each job in its own shape, on the mixed-shape kernel
+    # 16-row tiles and 6: each job's rows split over the three heights, on
+    # the mixed-shape kernel
     # three replays of the captured graph, with other device work between
     # them, all equal to the single calls
     b, views = batch_run
@@ -162,11 +207,11 @@ def test_batch_against_oracle(batch_run):
             random = O.Random(1, True)
         else:
             random = O.Random(0)
+        q = dict(query=d["qraw"][:nq], query_len=d["qle"][:nq],
+                 query_cluster=d["qcl"][:nq]) if nq else {}
         want, counts, nf, _ = O.query_db(d["raw"][:nr], kl, s64, 14, d["le"][:nr],
-                                         query=d["qraw"][:nq], query_len=d["qle"][:nq],
-                                         ref_cluster=d["rcl"][:nr], query_cluster=d["qcl"][:nq],
-                                         random=random, jaccard=(kind == "jaccard"),
-                                         want_counts=True)
+                                         ref_cluster=d["rcl"][:nr], random=random,
+                                         jaccard=(kind == "jaccard"), want_counts=True, **q)
         got = b.result(x)
         assert nf == 0
         if kind == "counts":
