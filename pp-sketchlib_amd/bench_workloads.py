@@ -342,6 +342,7 @@ def serve_headline(n_ref, seconds, max_rows, slots):
     q_raw, q_le = raw[n_ref:].copy(), le[n_ref:].copy()
     ref = H.SketchSet(raw[:n_ref], KL8, S64, lengths=le[:n_ref])
     del raw
+    db_bytes = n_ref * len(KL8) * S64 * 14 * 8
     setup_s = time.time() - t0
     print(f"# serving: {n_ref} reference sketches generated and resident in {setup_s:.1f}s",
           flush=True)
@@ -360,11 +361,13 @@ def serve_headline(n_ref, seconds, max_rows, slots):
     try:
         srv.wait(srv.submit(q_raw[:8], q_le[:8]))  # warm-up
         sat_req = draw(64)
+        st0 = srv.stats()
         t = time.perf_counter()
         tickets = [srv.submit(q_raw[s_:e_], q_le[s_:e_]) for s_, e_ in sat_req]
         for tk in tickets:
             srv.wait(tk)
         sat_s = time.perf_counter() - t
+        sat_steps = srv.stats()["steps"] - st0["steps"]
         sat_queries = sum(e_ - s_ for s_, e_ in sat_req)
         sat_rate = sat_queries * n_ref / sat_s
         sat_qps = len(sat_req) / sat_s
@@ -381,6 +384,12 @@ def serve_headline(n_ref, seconds, max_rows, slots):
         "p99_ms": round(float(np.percentile(lat, 99)), 2),
         "value": round(pairs / wall, 1), "unit": "distances/s",
         "saturation_value": round(sat_rate, 1),
+        # int32 matching work and reference-set passes at saturation: each
+        # step streams the resident set at least once (once per 64-row tile)
+        "saturation_int_ops_per_s": round(sat_rate * _ops_per_pair(len(KL8), S64), 1),
+        "saturation_valu_frac": round(sat_rate * _ops_per_pair(len(KL8), S64) / 39.3216e12, 4),
+        "saturation_set_bytes_per_s": round(sat_steps * db_bytes / sat_s, 1),
+        "saturation_steps": sat_steps,
         "load": "Poisson arrivals at 70% of the saturation request rate",
         "qps": round(qps, 3), "requests": len(reqs), "window_s": round(wall, 2),
         "n_fail": int(sum(r["n_fail"] for r in results)),
