@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdint>
 
+#include "bin_fraction.h"
 #include "log_jaccard.h"
 #include "ppsk_internal.h"
 
@@ -159,11 +160,15 @@ __device__ __forceinline__ void add_progress(const Control& c, unsigned long lon
   }
 }
 
-__device__ __forceinline__ double jaccard_d(uint32_t c, double bins, double r) {
-  const double obs = (double)c / bins;
+__device__ __forceinline__ double jaccard_d(uint32_t c, double bins, double rbins, double r) {
+  const double obs = bin_fraction(c, bins, rbins);
   const double diff = obs > r ? (obs - r) : 0.0;
   return (diff * 1.0) / (1.0 - r);
 }
+
+// Table of the regression's logarithm (log_jaccard.h); every device holds a
+// copy, written once when the library sets the device up.
+__device__ double g_log_tab[LOG_TAB_DOUBLES];
 
 
 // ---------------------------------------------------------------------------
@@ -599,6 +604,7 @@ __device__ __noinline__ void tile_epilogue(CParams* pp, const uint16_t* tile_cou
   constexpr int TC = Cfg::TC;
   constexpr int TROWS = Cfg::TROWS;
   const double bins = 64.0 * (double)p.s64;
+  const double rbins = 1.0 / bins;
   // this wave's rows and this lane's first column in the tile (wave grid)
   const int wrow = (wave / Cfg::CG) * RQ;
   const int wcol = (wave % Cfg::CG) * 64 * RR + lane;
@@ -635,7 +641,7 @@ __device__ __noinline__ void tile_epilogue(CParams* pp, const uint16_t* tile_cou
       if (p.out_kind == OUT_JACCARD) {
         float* o = reinterpret_cast<float*>(p.out) + pos * p.n_k;
         for (int k = 0; k < p.n_k; ++k) {
-          o[k] = (float)jaccard_d(cp[k * kstride], bins, random_for(p, k, gi, gj));
+          o[k] = (float)jaccard_d(cp[k * kstride], bins, rbins, random_for(p, k, gi, gj));
         }
         continue;
       }
@@ -644,7 +650,7 @@ __device__ __noinline__ void tile_epilogue(CParams* pp, const uint16_t* tile_cou
         // (src/api.cpp:368-371 narrows 1.0f - double J to float)
         float v[2] = {INFINITY, INFINITY};
         for (int k = 0; k < 2 && k < p.n_k; ++k) {
-          v[k] = (float)(1.0 - jaccard_d(cp[k * kstride], bins, random_for(p, k, gi, gj)));
+          v[k] = (float)(1.0 - jaccard_d(cp[k * kstride], bins, rbins, random_for(p, k, gi, gj)));
         }
         reinterpret_cast<float2*>(p.out)[pos] = make_float2(v[0], v[1]);
         continue;
@@ -655,11 +661,11 @@ __device__ __noinline__ void tile_epilogue(CParams* pp, const uint16_t* tile_cou
       int first = -1, last = -1;
       double sy = 0.0, sxy = 0.0, py = 0.0, pxy = 0.0;
       for (int k = 0; k < p.n_k; ++k) {
-        const float jf = (float)jaccard_d(cp[k * kstride], bins, random_for(p, k, gi, gj));
+        const float jf = (float)jaccard_d(cp[k * kstride], bins, rbins, random_for(p, k, gi, gj));
         const bool ok = (double)jf > p.tolerance;
         if (!ok && first < 0) continue;
         const double x = (double)p.kvals[k];
-        const double y = log_jaccard(jf);
+        const double y = log_jaccard(jf, g_log_tab);
         if (ok) {
           if (first < 0) {
             first = k;
@@ -1134,6 +1140,12 @@ __global__ void tile_reset_kernel(uint32_t* tile_done, int64_t n, unsigned long 
     fail[0] = 0ull;
     fail[1] = ~0ull;
   }
+}
+
+hipError_t upload_log_table() {
+  double tab[LOG_TAB_DOUBLES];
+  log_table_fill(tab);
+  return hipMemcpyToSymbol(HIP_SYMBOL(g_log_tab), tab, sizeof(tab));
 }
 
 hipError_t launch_tile_reset(uint32_t* tile_done, int64_t n, unsigned long long* fail,

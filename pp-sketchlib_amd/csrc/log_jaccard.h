@@ -4,37 +4,54 @@
 // a host build of this same header checks its accuracy (exhaustively over
 // the floats of [2^-12, 2], against a long-double log).
 //
-// x = m 2^e with m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s) with
-// s = (m - 1) / (m + 1), |s| <= 0.1716, summed through s^17 (the next term is
-// below 3e-16), plus e ln 2 with ln 2 split into a short head and a tail
-// (fdlibm's split) so that e * head is exact: about 1e-15 absolute, with a
-// third of the instructions of the library log.  Zero, subnormal and
-// non-finite values take the library log (ln 0 = -inf, as in the
-// reference's fit).
+// x = m 2^e with m in [1, 2).  The top 8 fraction bits of m pick one of 256
+// buckets [1 + i/256, 1 + (i+1)/256); the table holds c_i, the double nearest
+// 1 / (1 + (i + 1/2) / 256), and t_i, the double nearest -ln c_i (bucket 0:
+// c = 1, t = 0, so that ln 1 = 0 exactly).  Then ln m = t_i + ln(1 + u) with
+// u = m c_i - 1 (one fma; |u| <= 2^-9, 2^-8 in bucket 0), whose series is
+// summed through u^6 (the next term is below 2^-56 / 7), and
+// ln x = e ln 2 + t_i + ln(1 + u) with ln 2 split into a short head and a tail
+// (fdlibm's split) so that e * head is exact: about 1e-15 absolute over the
+// Jaccard range, with no division and no branch.  Zero gives -inf, negative
+// values and NaN give NaN, +inf gives +inf, as the library log does.
 #pragma once
 
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 
-__host__ __device__ inline double log_jaccard(float xf) {
-  if (!(xf >= 1.17549435e-38f && xf <= 3.40282347e+38f)) return log((double)xf);
-  int e = 0;
-  double m = frexp((double)xf, &e);  // [0.5, 1)
-  if (m < 0.70710678118654752440) {
-    m *= 2.0;
-    e -= 1;
+// Entries of the table: c_i at [2 i], t_i at [2 i + 1].
+constexpr int LOG_TAB_BUCKETS = 256;
+constexpr int LOG_TAB_DOUBLES = 2 * LOG_TAB_BUCKETS;
+
+// Host: fill the table (the device gets a copy of these same doubles).
+inline void log_table_fill(double* tab) {
+  for (int i = 0; i < LOG_TAB_BUCKETS; ++i) {
+    const double c = i == 0 ? 1.0 : 1.0 / (1.0 + (i + 0.5) / LOG_TAB_BUCKETS);
+    tab[2 * i] = c;
+    tab[2 * i + 1] = i == 0 ? 0.0 : (double)(-std::log((long double)c));
   }
-  const double sv = (m - 1.0) / (m + 1.0);
-  const double s2 = sv * sv;
-  double poly = 2.0 / 17.0;
-  poly = fma(poly, s2, 2.0 / 15.0);
-  poly = fma(poly, s2, 2.0 / 13.0);
-  poly = fma(poly, s2, 2.0 / 11.0);
-  poly = fma(poly, s2, 2.0 / 9.0);
-  poly = fma(poly, s2, 2.0 / 7.0);
-  poly = fma(poly, s2, 2.0 / 5.0);
-  poly = fma(poly, s2, 2.0 / 3.0);
-  poly = fma(poly, s2, 2.0);
-  const double lnm = sv * poly;
-  const double ed = (double)e;
-  return fma(ed, 6.93147180369123816490e-01, fma(ed, 1.90821492927058770002e-10, lnm));
+}
+
+__host__ __device__ inline double log_jaccard(float xf, const double* tab) {
+  const double x = (double)xf;
+  int e = 0;
+  const double m = 2.0 * frexp(x, &e);  // [1, 2) for finite nonzero x
+  uint64_t bits;
+  std::memcpy(&bits, &m, sizeof(bits));
+  const int i = (int)((bits >> 44) & (LOG_TAB_BUCKETS - 1));  // in range for any x
+  const double c = tab[2 * i];
+  const double t = tab[2 * i + 1];
+  const double u = fma(m, c, -1.0);
+  double p = fma(-1.0 / 6.0, u, 0.2);
+  p = fma(p, u, -0.25);
+  p = fma(p, u, 1.0 / 3.0);
+  p = fma(p, u, -0.5);
+  p = fma(p, u, 1.0);
+  const double lm = fma(u, p, t);  // ln m
+  const double ed = (double)(e - 1);
+  double y = fma(ed, 6.93147180369123816490e-01, fma(ed, 1.90821492927058770002e-10, lm));
+  if (!(xf > 0.0f)) y = xf == 0.0f ? -INFINITY : NAN;
+  if (xf == INFINITY) y = INFINITY;
+  return y;
 }

@@ -53,6 +53,7 @@ int get_ctx(int device, DeviceCtx** out) {
     HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&c->fail, 2 * sizeof(unsigned long long)));
     if (int rc = noalloc_init(&c->noalloc_scratch)) return rc;
+    HIP_TRY(ppsk::upload_log_table());
     // progress / cancellation words: a device counter and two host-mapped,
     // coherent words (device_id < MAX_DEVICES here)
     HIP_TRY(hipMalloc(&c->ctl.progress, sizeof(unsigned long long)));

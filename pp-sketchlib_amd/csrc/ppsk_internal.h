@@ -142,6 +142,9 @@ hipError_t launch_batch_reset(uint32_t* queue_next, unsigned long long* fail, in
                               hipStream_t stream, unsigned long long* progress = nullptr);
 // Zeroes n tile completion counters and sets fail ([2]: count, first row; may
 // be null) to (0, max), with a kernel.
+// Writes the table of the distance epilogue's logarithm into the current
+// device's copy (once per device, before its first distance kernel).
+hipError_t upload_log_table();
 hipError_t launch_tile_reset(uint32_t* tile_done, int64_t n, unsigned long long* fail,
                              hipStream_t stream);
 
