@@ -66,7 +66,8 @@ def _check_pipelined_spills(remarks):
         if "Function Name:" in line:
             name = line.split("Function Name:")[1].split()[0]
         elif name and ("SGPRs Spill:" in line or "VGPRs Spill:" in line):
-            pipelined = ("dist_tiles" in name or "batch_tile_k" in name) and "ELi2EE" in name
+            pipelined = (("dist_tiles" in name or "batch_tile_k" in name) and "ELi2EE" in name
+                         or "dist_tiles_diagonal" in name)
             if pipelined and int(line.rsplit(":", 1)[1].split()[0]) != 0:
                 bad.append(name + ": " + line.split("remark:")[-1].strip())
     if bad:

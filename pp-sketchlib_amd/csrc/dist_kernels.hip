@@ -716,10 +716,130 @@ __device__ __noinline__ void tile_epilogue(CParams* pp, const uint16_t* tile_cou
 // pp: the distance parameters, read with scalar loads (kernarg segment or a
 // job table entry).  sB0 / sB1: the two LDS stage buffers.
 // ---------------------------------------------------------------------------
+// The stage loop of tile_match with a narrower matching, for the diagonal
+// tiles of a self pass (tile_match_diagonal below).  Copies and the row
+// prefetch follow the tile's shape <RQ, RR>; the matching covers MR column
+// blocks of 64 per lane from tile column `col`, against the same column
+// stage (same plane stride and words per stage).  tile_match keeps its own
+// copy of this loop so that the code of the kernels without the diagonal
+// path stays exactly as it is.
+template <int RQ, int RR, int PB, int MR>
+__device__ __forceinline__ void tile_stages(const CU32* arow, const uint32_t* cbase,
+                                            const ColOffsets<RQ, RR>& off,
+                                            const uint32_t* rbase, const RowOffsets<RQ, RR>& roff,
+                                            uint32_t rdst, bool rpf, int s64, uint4* sB0,
+                                            uint4* sB1, int wave, int lane, int col,
+                                            uint32_t (&cnt)[RQ][MR]) {
+  constexpr int WS = TileCfg<RQ, RR>::WS;
+  static_assert(MR == RR || (PB != 0 && TileCfg<RQ, MR>::TC == TileCfg<RQ, RR>::TC &&
+                             TileCfg<RQ, MR>::WS == WS),
+                "a narrower matching must read the same column stage");
+#pragma unroll
+  for (int q = 0; q < RQ; ++q)
+#pragma unroll
+    for (int r = 0; r < MR; ++r) cnt[q][r] = 0;
+
+  const int n_stages = (s64 + WS - 1) / WS;
+  issue_stage<RQ, RR>(cbase, off, sB0, wave);
+  if (rpf) prefetch_round<RQ, RR, 0>(rbase, roff, rdst, wave);
+  // One barrier per stage: after it, stage s is in LDS for every wave and
+  // every wave has finished matching stage s - 1, whose buffer the copy of
+  // stage s + 1 (issued right after the barrier) then overwrites while stage
+  // s is matched.  Stages alternate between the two buffers; each pass of
+  // this loop runs an even stage from sB0 and the following odd stage from
+  // sB1.
+  for (int st = 0; st < n_stages; st += 2) {
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int s = st + half;
+      if (s < n_stages) {
+        uint4* cur = half ? sB1 : sB0;
+        uint4* nxt = half ? sB0 : sB1;
+        const int w0 = s * WS;
+        wait_vmcnt<0>();
+        stage_barrier();
+        if (s + 1 < n_stages) {
+          issue_stage<RQ, RR>(cbase + (int64_t)(w0 + WS) * (HL * BLK), off, nxt, wave);
+          if (rpf) prefetch_round<RQ, RR, 0>(rbase + (int64_t)(w0 + WS) * (HL * BLK), roff, rdst, wave);
+        }
+        const int nw = (s64 - w0) < WS ? (s64 - w0) : WS;
+        if constexpr (PB == 0) {
+          match_stage_plain<RQ, MR>(arow, w0, nw, reinterpret_cast<const uint32_t*>(cur), lane,
+                                    cnt);
+        } else {
+          match_stage_pipe<RQ, MR, PB>(arow, w0, nw, cur, col, cnt);
+        }
+      }
+    }
+  }
+  // every wave is done with both buffers before they are reused
+  stage_barrier();
+}
+
+// Diagonal tiles of a self pass (upper triangle, i < j): a 64-column block
+// of a 64 x 256 tile whose last column precedes the tile's first row holds no
+// output pair.  A tile whose first row lies 128 (192) columns past its first
+// column matches only its last 2 (1) blocks, with the matching of the
+// 32-row (16-row) shape, which reads the same column stage; the skipped
+// counts are zero and the epilogue never reads them.  Returns false (and
+// does nothing) for any other tile.  Tile-uniform.
 template <int RQ, int RR, int PB>
+__device__ __forceinline__ bool tile_match_diagonal(CParams* pp, Tile tile, int kk, uint4* sB0,
+                                                    uint4* sB1, uint4* sR, int wave, int lane,
+                                                    uint32_t (&cnt)[RQ][RR]) {
+  static_assert(RQ == 8 && RR == 4 && PB != 0, "64 x 256 pipelined tiles only");
+  using Cfg = TileCfg<RQ, RR>;
+  if (!pp->self_mode || pp->full_square) return false;
+  const int64_t row0 = (int64_t)tile.bi * Cfg::TROWS;
+  const int64_t col0 = (int64_t)tile.ct * Cfg::TC;
+  const int64_t d = row0 - col0;
+  if (d < 2 * 64) return false;
+  const int s64 = pp->s64;
+  const CU32* arow = (CU32*)(pp->row_stripes +
+                             ((int64_t)kk * pp->nb_rows + row0 / BLK) * (int64_t)s64 * (HL * BLK) +
+                             row0 % BLK + (wave / Cfg::CG) * RQ);
+  const int nb_cols = pp->nb_cols;
+  const uint32_t* cbase = pp->col_stripes + (int64_t)kk * nb_cols * (int64_t)s64 * (HL * BLK) +
+                          (int64_t)(col0 / BLK) * s64 * (HL * BLK);
+  const ColOffsets<RQ, RR> off = col_offsets<RQ, RR>(col0, nb_cols, s64, wave, lane);
+  const bool rpf = pp->row_prefetch != 0;
+  const uint32_t* rbase = pp->row_stripes +
+                          ((int64_t)kk * pp->nb_rows + row0 / BLK) * (int64_t)s64 * (HL * BLK) +
+                          row0 % BLK;
+  const RowOffsets<RQ, RR> roff = row_offsets<RQ, RR>(wave, lane);
+  const uint32_t rdst = lds_addr(sR) + (uint32_t)(wave * 64 * 16);
+  if (d >= 3 * 64) {
+    uint32_t c1[RQ][1];
+    tile_stages<RQ, RR, PB, 1>(arow, cbase, off, rbase, roff, rdst, rpf, s64, sB0, sB1, wave,
+                               lane, lane + 3 * 64, c1);
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+      cnt[q][0] = cnt[q][1] = cnt[q][2] = 0;
+      cnt[q][3] = c1[q][0];
+    }
+  } else {
+    uint32_t c2[RQ][2];
+    tile_stages<RQ, RR, PB, 2>(arow, cbase, off, rbase, roff, rdst, rpf, s64, sB0, sB1, wave,
+                               lane, lane + 2 * 64, c2);
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+      cnt[q][0] = cnt[q][1] = 0;
+      cnt[q][2] = c2[q][0];
+      cnt[q][3] = c2[q][1];
+    }
+  }
+  return true;
+}
+
+// SKIP: diagonal tiles of a self pass go through tile_match_diagonal (only
+// the kernels instantiated with it carry that code).
+template <int RQ, int RR, int PB, bool SKIP = false>
 __device__ __forceinline__ void tile_match(CParams* pp, Tile tile, int kk, uint4* sB0,
                                            uint4* sB1, uint4* sR, int wave, int lane,
                                            uint32_t (&cnt)[RQ][RR]) {
+  if constexpr (SKIP) {
+    if (tile_match_diagonal<RQ, RR, PB>(pp, tile, kk, sB0, sB1, sR, wave, lane, cnt)) return;
+  }
   using Cfg = TileCfg<RQ, RR>;
   constexpr int TC = Cfg::TC;
   constexpr int WS = Cfg::WS;
@@ -842,6 +962,62 @@ __global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles(DistParams p) {
   if (!stop) {
     uint32_t cnt[RQ][RR];
     tile_match<RQ, RR, PB>(pp, tile, kk, sB0, sB1, sR, wave, lane, cnt);
+    store_counts<RQ, RR>(tile_counts, kk, wave, lane, cnt);
+  }
+
+  // completion: the last workgroup of this tile runs the epilogue (unless
+  // the call was cancelled, which then holds for every later read)
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned prev = atomicAdd(&p.tile_done[t], 1u);
+    const bool last = prev == (unsigned)(p.n_k - 1);
+    s_last = last ? (cancel_seen(p.ctl.cancel) ? 2 : 1) : 0;
+    if (last) p.tile_done[t] = 0;  // ready for the next launch
+  }
+  __syncthreads();
+  if (s_last != 1) return;
+  __threadfence();
+  tile_epilogue<RQ, RR>(pp, tile_counts, tile, wave, lane);
+  if (threadIdx.x == 0) add_progress(p.ctl, (unsigned long long)p.n_k);
+}
+
+// dist_tiles<8, 4, 2> for the (tile, k) items of a self pass's diagonal tiles:
+// the column blocks that hold no output pair are skipped (tile_match_diagonal);
+// any other tile runs as in dist_tiles.  A kernel of its own, so that the code
+// of dist_tiles itself does not change with it.
+__global__ __launch_bounds__(WAVES * 64, 4) void dist_tiles_diagonal(DistParams p) {
+  constexpr int RQ = 8, RR = 4, PB = 2;
+  using Cfg = TileCfg<RQ, RR>;
+  __shared__ uint4 sB0[Cfg::STAGE_U4];
+  __shared__ uint4 sB1[Cfg::STAGE_U4];
+  __shared__ uint4 sR[PB ? Cfg::ROW_LDS_U4 : 64];  // row prefetch: pipelined shapes only
+  __shared__ int s_last;
+  __shared__ int s_stop;
+  // the parameters are the kernel's only argument: scalar loads straight
+  // from the kernarg segment (no copy)
+  CParams* pp = (CParams*)__builtin_amdgcn_kernarg_segment_ptr();
+
+  const int64_t item = p.item_base + blockIdx.x;
+  const int64_t t = item / p.n_k;
+  const int kk = (int)(item - t * p.n_k);
+  Tile tile = p.tiles[t];
+  // wave-uniform values (kept in SGPRs)
+  tile.bi = __builtin_amdgcn_readfirstlane(tile.bi);
+  tile.ct = __builtin_amdgcn_readfirstlane(tile.ct);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  // a cancelled call skips the matching; one read per workgroup, shared, so
+  // that every wave takes the same branch (the matching has barriers)
+  if (threadIdx.x == 0) s_stop = cancel_seen(p.ctl.cancel) ? 1 : 0;
+  __syncthreads();
+  const int stop = __builtin_amdgcn_readfirstlane(s_stop);
+
+  // per-tile count scratch: [tile][kk][row][col]
+  uint16_t* tile_counts = p.counts + t * p.n_k * (int64_t)(Cfg::TROWS * Cfg::TC);
+  if (!stop) {
+    uint32_t cnt[RQ][RR];
+    tile_match<RQ, RR, PB, true>(pp, tile, kk, sB0, sB1, sR, wave, lane, cnt);
     store_counts<RQ, RR>(tile_counts, kk, wave, lane, cnt);
   }
 
@@ -1140,6 +1316,12 @@ __global__ void tile_reset_kernel(uint32_t* tile_done, int64_t n, unsigned long 
     fail[0] = 0ull;
     fail[1] = ~0ull;
   }
+}
+
+hipError_t launch_dists_diagonal(const DistParams& p, int64_t n_items, hipStream_t stream) {
+  if (n_items <= 0) return hipSuccess;
+  dist_tiles_diagonal<<<(unsigned)n_items, WAVES * 64, 0, stream>>>(p);
+  return hipGetLastError();
 }
 
 hipError_t upload_log_table() {

@@ -405,8 +405,8 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
   // the finest streaming granularity
   const bool finest = host_out && std::getenv("PPSK_CHUNK_BYTES");
   std::vector<size_t> gs;
-  const std::vector<ppsk::Tile> tiles = make_tiles(p.n_rows, p.n_cols, tr, tc, upper, row_begin,
-                                                   row_end, finest ? 1 : tile_group(tr), &gs);
+  std::vector<ppsk::Tile> tiles = make_tiles(p.n_rows, p.n_cols, tr, tc, upper, row_begin,
+                                             row_end, finest ? 1 : tile_group(tr), &gs);
   if (tiles.empty()) return PPSK_OK;
   // 16-row tiles run on the mixed-shape kernel (every item in that shape)
   const int kernel = variant == 5 ? ppsk::BATCH_MIXED : variant;
@@ -427,6 +427,21 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
   // of the slots, those tiles go to a second launch as (tile, k-mer length)
   // items of the per-(tile, k) kernel instead (PPSK_TAIL_SPLIT=0 turns this
   // off).  Same tile code, same epilogue, same output.
+  //
+  // In a self pass (upper triangle) of 64 x 256 tiles, a diagonal tile whose
+  // first row lies at least 128 columns past its first column has its first
+  // 2 or 3 column blocks below the diagonal: each chunk's such tiles are
+  // moved to its end, into the tail, whose kernel skips those blocks
+  // (dist_tiles_diagonal); at 8,192 samples they are exactly the tail.
+  const bool diagonal = variant == 2 && self && upper && p.self_mode && !p.full_square;
+  if (diagonal) {
+    for (const auto& ch : chunks) {
+      std::stable_partition(tiles.begin() + ch.first, tiles.begin() + ch.second,
+                            [&](const ppsk::Tile& t) {
+                              return (int64_t)t.bi * tr - (int64_t)t.ct * tc < 2 * 64;
+                            });
+    }
+  }
   std::vector<size_t> tail(chunks.size(), 0);
   std::vector<ppsk::Tile> tail_tiles;
   std::vector<size_t> tail_at(chunks.size(), 0);
@@ -513,7 +528,11 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
       pt.item_base = 0;
       hipStream_t ts = tail_beside ? c->tail_stream : c->stream;
       if (tail_beside) HIP_TRY(hipStreamWaitEvent(ts, e0, 0));
-      HIP_TRY(ppsk::launch_dists(pt, variant, (int64_t)tail[ci] * p.n_k, ts));
+      if (diagonal) {
+        HIP_TRY(ppsk::launch_dists_diagonal(pt, (int64_t)tail[ci] * p.n_k, ts));
+      } else {
+        HIP_TRY(ppsk::launch_dists(pt, variant, (int64_t)tail[ci] * p.n_k, ts));
+      }
       if (tail_beside) {
         HIP_TRY(hipEventRecord(et, ts));
         HIP_TRY(hipStreamWaitEvent(c->stream, et, 0));

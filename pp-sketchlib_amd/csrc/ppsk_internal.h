@@ -142,6 +142,9 @@ hipError_t launch_batch_reset(uint32_t* queue_next, unsigned long long* fail, in
                               hipStream_t stream, unsigned long long* progress = nullptr);
 // Zeroes n tile completion counters and sets fail ([2]: count, first row; may
 // be null) to (0, max), with a kernel.
+// launch_dists for 64 x 256 tiles (variant 2) of a self pass (upper
+// triangle): diagonal tiles skip the column blocks that hold no output pair.
+hipError_t launch_dists_diagonal(const DistParams& p, int64_t n_items, hipStream_t stream);
 // Writes the table of the distance epilogue's logarithm into the current
 // device's copy (once per device, before its first distance kernel).
 hipError_t upload_log_table();

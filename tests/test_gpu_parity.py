@@ -251,7 +251,10 @@ def test_row_prefetch_off_is_identical(monkeypatch):
 def test_tail_split_persistent_pass(monkeypatch, variant):
     # a pass just over one round of whole tiles: its partial last round runs
     # as (tile, k) items of the per-(tile, k) kernel; outputs are identical to
-    # the whole-tile run and to the oracle
+    # the whole-tile run and to the oracle.  With 64 x 256 tiles (variant 2)
+    # the tail of this self pass is its diagonal tiles whose first row lies 128
+    # or 192 columns past their first column (31 of them at 4,000 samples),
+    # which dist_tiles_diagonal runs with their leading column blocks skipped
     monkeypatch.setenv("PPSK_TILE_VARIANT", str(variant))
     monkeypatch.setenv("PPSK_PERSISTENT", "1")
     # just over one round of the 512 resident workgroups
