@@ -255,7 +255,7 @@ def main():
         "unit": "TOP/s",
         "frac": round(achieved / PEAK_TOPS, 4),
         "traffic": traffic_from_profile(),
-        "kernel": "dist_batch + dist_tiles (last partial round)",
+        "kernel": "dist_batch + dist_tiles_diagonal (last partial round: the diagonal tiles)",
         "kernel_ms": round(pass_ms, 3),
         "launches_per_step": round(launches / a.steps, 2),
         "ops_per_pair": OPS_PER_PAIR,
