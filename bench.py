@@ -56,7 +56,7 @@ def parse():
                     help="CPU baseline runs per thread count (the median is reported)")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving measurement of the headline line")
-    ap.add_argument("--headline-serve-seconds", type=float, default=10.0,
+    ap.add_argument("--headline-serve-seconds", type=float, default=20.0,
                     help="Poisson arrival window of the headline line's serving figure")
     ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                     help="run one plain copy kernel of BYTES and exit (HBM counter calibration)")
