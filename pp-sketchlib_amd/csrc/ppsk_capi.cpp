@@ -432,14 +432,18 @@ int run_dense_persistent(DeviceCtx* c, ppsk::DistParams p, int variant, bool upp
   // first row lies at least 128 columns past its first column has its first
   // 2 or 3 column blocks below the diagonal: each chunk's such tiles are
   // moved to its end, into the tail, whose kernel skips those blocks
-  // (dist_tiles_diagonal); at 8,192 samples they are exactly the tail.
+  // (dist_tiles_diagonal); at 8,192 samples they are exactly the tail.  The
+  // tiles that keep 2 blocks come before those that keep 1, so that the
+  // items dispatched last, into the slots freed last, are the shortest.
   const bool diagonal = variant == 2 && self && upper && p.self_mode && !p.full_square;
   if (diagonal) {
+    auto kept = [&](const ppsk::Tile& t) {  // 0: whole tile; 1: 2 blocks; 2: 1 block
+      const int64_t d = (int64_t)t.bi * tr - (int64_t)t.ct * tc;
+      return d >= 3 * 64 ? 2 : (d >= 2 * 64 ? 1 : 0);
+    };
     for (const auto& ch : chunks) {
-      std::stable_partition(tiles.begin() + ch.first, tiles.begin() + ch.second,
-                            [&](const ppsk::Tile& t) {
-                              return (int64_t)t.bi * tr - (int64_t)t.ct * tc < 2 * 64;
-                            });
+      std::stable_sort(tiles.begin() + ch.first, tiles.begin() + ch.second,
+                       [&](const ppsk::Tile& a, const ppsk::Tile& b) { return kept(a) < kept(b); });
     }
   }
   std::vector<size_t> tail(chunks.size(), 0);
